@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Benchmark: attempted spin updates/s of the parallel-tempering Metropolis sweep on one MI355X.
+
+Workload (default ``--config pt_fcc16``): FCC conventional cell, 16^3 cells (N = 16384 FP64
+3-vector spins), J1 = -1, J2 = +0.2, J3 = +0.05, easy-axis single-ion Delta = -0.1 along z,
+64 replicas on a geometric ladder 0.05-1.5 J, replica exchange after every sweep,
+'random' proposals, uniform-on-sphere initial spins (seed 4242).  One step = one sweep of
+every replica + one measurement of every replica + one exchange round.  Synthetic inputs,
+resident in HBM before the timed region.
+
+Prints one JSON line (stdout).  Extra legs:
+  roofline      -- sweep-kernel bytes per launch (algorithmic, gather model) / mean launch
+                   time, timed with HIP events on the engine's stream
+  cpu_baseline  -- the oracle's reference-order sweep (MT19937, sequential sites, FP64),
+                   one process per host core, on a bounded sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "attempted spin updates/s (all replicas); effective samples/s of E vs plain"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(name):
+    from espins_amd import lattice as Lt
+    if name == "pt_fcc16":
+        m, tems = Lt.config_c2(L=16, R=64)
+        desc = ("PT FCC 16^3 cells (N=16384), J1=-1 J2=+0.2 J3=+0.05, Delta=-0.1 (z), "
+                "64 replicas geometric T 0.05-1.5 J, exchange every sweep")
+        return m, tems, desc, 4242
+    if name == "mc_sc8":
+        m, tems = Lt.config_c1(L=8)
+        return m, tems, "plain MC SC 8^3 (N=512), J1=1, 16 T lin 0.5-2.0", 2024
+    if name == "pt_sc64_dm":
+        m, tems = Lt.config_c4(L=64, R=32)
+        return m, tems, "PT SC 64^3 (N=262144) J1=1 + DM 0.3 + B 0.05 + Delta=+0.02, 32 replicas", 77
+    if name == "pt_fcc32":
+        m, tems = Lt.config_c5(L=32, R=64)
+        return m, tems, "PT FCC 32^3 cells (N=131072) J1=-1 J2=0.1 B=0.1 Delta=-0.05, 64 replicas", 5
+    raise SystemExit(f"unknown config {name}")
+
+
+def bytes_per_update(m):
+    """Algorithmic HBM bytes per attempted update, CSR/ELL gather model:
+    52 + sum over distinct bonds of (28 + 8 p_b), p_b = FP64 coupling scalars on the bond
+    (J 1, B 1, D 3); own spin read+write 48 B + row offset 4 B; per bond neighbour spin 24 B +
+    int32 index 4 B."""
+    import numpy as np
+    N = m.N
+    tot = 0.0
+    for i in range(0, N, max(1, N // 512)):     # sample of sites (periodic lattices)
+        bonds = {}
+        for tab, p in ((m.J, 1), (m.B, 1), (m.D, 3)):
+            if tab is None:
+                continue
+            for l in range(int(tab.cnt[i])):
+                j = int(tab.nbr[l, i])
+                bonds[j] = bonds.get(j, 0) + p
+        tot += 52 + sum(28 + 8 * p for p in bonds.values())
+    return tot / len(range(0, N, max(1, N // 512)))
+
+
+# ---------------------------------------------------------------- CPU baseline ------------
+def _cpu_worker(args):
+    name, T, seconds, seed = args
+    import numpy as np
+    from espins_amd import lattice as Lt
+    from oracle import oracle as O
+    if name == "pt_fcc16":
+        m = Lt.cubic_model("fcc", 16, J=(-1.0, 0.2, 0.05), K=-0.1, builder=O.build_table_fn)
+    elif name == "mc_sc8":
+        m = Lt.cubic_model("sc", 8, J=(1.0,), builder=O.build_table_fn)
+    elif name == "pt_sc64_dm":
+        m = Lt.cubic_model("sc", 64, J=(1.0,), B=0.05, D=0.3, K=0.02, builder=O.build_table_fn)
+    else:
+        m = Lt.cubic_model("fcc", 32, J=(-1.0, 0.1), B=0.1, K=-0.05, builder=O.build_table_fn)
+    o = O.Oracle(m)
+    s = Lt.random_spins(m.N, 1, seed)[0]
+    st = O.mt_state(seed)
+    e = o.energy(s)
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        e, _ = o.sweep_ref(s, 1.0 / T, 0, 0.125, st, 1, e)
+        n += 1
+    return n * m.N, time.perf_counter() - t0
+
+
+def cpu_baseline(name, tems, seconds):
+    import multiprocessing as mp
+    try:
+        ncores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncores = os.cpu_count() or 1
+    ncores = max(1, min(ncores, 16))
+    jobs = [(name, float(tems[k % len(tems)]), seconds, 1000 + k) for k in range(ncores)]
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(ncores) as pool:
+        res = pool.map(_cpu_worker, jobs)
+    wall = time.perf_counter() - t0
+    upd = sum(r[0] for r in res)
+    busy = max(r[1] for r in res)
+    return dict(value=upd / busy, unit="updates/s", cores=ncores, kind="port",
+                sample=(f"oracle reference-order Metropolis (MT19937, sites 1..N, FP64) on the "
+                        f"same Hamiltonian, one process per core, one temperature each, "
+                        f"{seconds:.0f} s per core ({upd} updates, {wall:.1f} s wall)"))
+
+
+# ---------------------------------------------------------------- main ---------------------
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--config", default="pt_fcc16")
+    ap.add_argument("--path", type=int, default=0, help="0 auto, 1 global-memory phases, 2 LDS")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus must be 1 "
+              "(multi-GPU sharding of replica ladders is described in DESIGN.md, not built)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    m, tems, desc, seed = workload(args.config)
+    cpu = None
+    if not args.no_cpu:   # before any GPU call (forked workers must not inherit a GPU context)
+        log(f"cpu_baseline: {args.cpu_seconds:.0f} s per core ...")
+        cpu = cpu_baseline(args.config, tems, args.cpu_seconds)
+        log(f"cpu_baseline: {cpu['value'] / 1e6:.2f} M updates/s on {cpu['cores']} cores")
+
+    import numpy as np
+    from espins_amd import lattice as Lt
+    from espins_amd.engine import Engine
+    try:
+        import torch
+        has_torch = torch.cuda.is_available()
+    except Exception:
+        has_torch = False
+
+    R = len(tems)
+    eng = Engine(0)
+    eng.set_model(m)
+    eng.set_path(args.path)
+    eng.set_replicas(tems, Lt.random_spins(m.N, R, seed, m.mag))
+    info = eng.model_info()
+    log(f"model: {info}")
+    pt = R > 1 and args.config.startswith("pt")
+    graph = not args.no_graph
+
+    def run(n, sweep0, step0):
+        eng.run(n, steps_per_round=1, steps_measure=1, measure=True, exchange=pt,
+                start_parity=1 + (step0 % 2), mode="random", seed=seed, sweep0=sweep0,
+                step0=step0, graph=graph)
+
+    run(args.warmup, 0, 0)
+    eng.reset_stats()
+    eng.sync()
+    if has_torch:
+        torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(args.steps, args.warmup, args.warmup)
+    eng.sync()
+    if has_torch:
+        torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    updates = m.N * R * args.steps
+    value = updates / dt
+    st = eng.stats()
+    acc_rate = 1.0 - st["rejected"].sum() / float(updates)
+    sr, ns = eng.pt_map()
+    swap_rate = float(ns[:, 1].sum()) / max(1, ns[:, 0].sum())
+    log(f"timed: {args.steps} steps in {dt:.3f} s -> {value / 1e9:.3f} G updates/s; "
+        f"acceptance {acc_rate:.4f}, swap acceptance {swap_rate:.4f}")
+
+    # sweep-kernel roofline (HIP events around each launch on the engine's stream)
+    nsw = max(10, min(200, args.steps // 10))
+    tm = eng.time_sweeps(nsw, mode="random", seed=seed + 1, sweep0=10 ** 9)
+    B = bytes_per_update(m)
+    ms_launch = tm["kernel_ms"] / tm["nlaunch"]
+    alg_bytes = tm["updates_per_launch"] * B
+    achieved = alg_bytes / (ms_launch * 1e-3) / 1e9
+    kernel = "k_lds" if info["lds"] else "k_phase"
+    log(f"{kernel}: {tm['nlaunch']} launches, mean {ms_launch * 1e3:.2f} us, "
+        f"{tm['updates_per_launch']} updates/launch, B_upd {B:.0f} B -> {achieved:.0f} GB/s")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": desc, "N": m.N, "replicas": R, "sweeps_per_step": 1,
+                   "exchange_every": 1 if pt else 0, "mode": "random", "colours": info["ncol"],
+                   "kernel_path": "lds" if info["lds"] else "global", "graph": graph},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": kernel, "bytes_per_update": B,
+                     "updates_per_launch": tm["updates_per_launch"],
+                     "mean_launch_us": ms_launch * 1e3},
+        "cpu_baseline": cpu,
+        "acceptance": acc_rate,
+        "swap_acceptance": swap_rate,
+    }
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

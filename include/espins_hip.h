@@ -1,0 +1,156 @@
+/*
+ * espins_hip.h -- C ABI of libespins_hip.so, the MI355X (gfx950) engine for the ESpinS
+ * Metropolis sweep and parallel-tempering replica exchange.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an int
+ * status (ESP_OK = 0, negative on error; esp_last_error() gives the text) and never
+ * aborts the process.  Arrays use the reference's Fortran layouts, so a Fortran caller
+ * passes its own arrays through ISO_C_BINDING without copies:
+ *   spin_matrix(3,N[,R])     double[R][N][3]
+ *   jij_matrix(N,z)          double[z][N]     (slot-major, site fastest)
+ *   jij_nbors_matrix(N,z)    int[z][N]        1-based site ids
+ *   jij_num_nbors_matrix(N)  int[N]
+ *   dij_matrix(3,N,z)        double[z][N][3]
+ * Energies and temperatures are in the reference's internal units (Kelvin: couplings
+ * already divided by k_B and by |m_i m_j|, as jij.F90:163,290 leave them).
+ *
+ * Threading: one host thread per context; calls on a context are ordered on its own
+ * HIP stream and are synchronous at return unless their name ends in _async.
+ */
+#ifndef ESPINS_HIP_H
+#define ESPINS_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ESP_OK        0
+#define ESP_EINVAL   -1   /* bad argument (null pointer, size out of range)      */
+#define ESP_EHIP     -2   /* HIP runtime error (no device, launch failure, ...)  */
+#define ESP_ESTATE   -3   /* call out of order (no model / no replicas yet)      */
+#define ESP_ENOMEM   -4   /* device or host allocation failed                     */
+#define ESP_ETABLE   -5   /* coupling rows name an unknown atomic position        */
+
+#define ESP_MODE_RANDOM 0 /* mcarlo_mode = random (get_quant.F90:80-93)          */
+#define ESP_MODE_CONST  1 /* mcarlo_mode = const  (get_quant.F90:94-161)         */
+
+typedef struct esp_ctx esp_ctx;
+
+/* ---- lifetime ------------------------------------------------------------------ */
+/* Replaces the module-global state of mc_parameters/mc_jij (parameters.F90:93,
+ * jij.F90:32-35): one opaque context per engine instance on HIP device `device`. */
+int esp_create(esp_ctx **ctx, int device);
+int esp_destroy(esp_ctx *ctx);
+const char *esp_last_error(const esp_ctx *ctx);
+/* Library version as 10000*major + 100*minor + patch. */
+int esp_version(void);
+
+/* ---- coupling tables (host only; no GPU needed) ---------------------------------- */
+/* jij_get / bij_get / dij_get (jij.F90:123-446, bij.F90:88-359, dij.F90:87-391):
+ * expand per-cell coupling rows over an L[0] x L[1] x L[2] supercell into ELL tables.
+ * basis_frac[nb][3]: basis positions (species-sorted order the site index uses);
+ * basis_mag[nb]: moments; rows: f1[nrows][3], f2[nrows][3], value[nrows] (internal units),
+ * dvec[nrows][3] (unit DM vectors, ncomp==3 only, else NULL).  Zero bonds are dropped.
+ * Outputs need capacity N*zcap (val: N*zcap*ncomp).  *zmax receives the realised width. */
+int esp_build_table(int nb, const double *basis_frac, const double *basis_mag,
+                    const int *L, int open_boundary,
+                    int nrows, const double *f1, const double *f2, const double *value,
+                    const double *dvec, int ncomp, int zcap,
+                    int *cnt, int *nbr, double *val, int *zmax);
+
+/* ---- model ------------------------------------------------------------------------ */
+/* Upload the Hamiltonian (the tables energy_get / delta_energy_get read,
+ * get_quant.F90:186-337).  zB = 0 / zD = 0 / have_si = 0 / have_field = 0 switch a term
+ * off (their pointers may then be NULL).  si_par/h_par[nb], si_axis/h_axis[nb][3].
+ * Computes a proper colouring of the union of the J, B and D graphs. */
+int esp_set_model(esp_ctx *ctx, int N, int nb, const double *mag,
+                  int zJ, const int *j_cnt, const int *j_nbr, const double *j_val,
+                  int zB, const int *b_cnt, const int *b_nbr, const double *b_val,
+                  int zD, const int *d_cnt, const int *d_nbr, const double *d_vec,
+                  int have_si, const double *si_par, const double *si_axis,
+                  int have_field, const double *h_par, const double *h_axis);
+/* Optional, before esp_set_model: the supercell_size (parameters.F90:352-357) the tables
+ * were expanded over.  Lets the colouring use a period of the lattice (colour classes of
+ * equal size, few colours); any result is verified and greedy colouring is the fallback. */
+int esp_set_supercell(esp_ctx *ctx, const int *L3);
+/* Host-only planning (no device needed): the colouring and layout facts esp_set_model
+ * would compute for these tables (L may be NULL).  colour[N] as in esp_get_coloring,
+ * info[5] as in esp_get_model_info (info[4] = 0). */
+int esp_plan_model(int N, int nb, const double *mag,
+                   int zJ, const int *j_cnt, const int *j_nbr, const double *j_val,
+                   int zB, const int *b_cnt, const int *b_nbr, const double *b_val,
+                   int zD, const int *d_cnt, const int *d_nbr, const double *d_vec,
+                   const int *L3, int *ncol, int *colour, int *info);
+/* Sweep kernel choice: 0 = automatic (LDS-resident when a replica fits in LDS), 1 = global-
+ * memory colour phases, 2 = LDS-resident (EINVAL if the replica does not fit). */
+int esp_set_path(esp_ctx *ctx, int path);
+/* colour[N]: 0-based colour of 1-based site i at colour[i-1]; *ncol colours. */
+int esp_get_coloring(esp_ctx *ctx, int *ncol, int *colour);
+/* Model facts: out[0]=N, out[1]=merged bond width, out[2]=colours, out[3]=1 if the
+ * couplings were compressed to per-basis tables, out[4]=1 if the LDS-resident sweep is used. */
+int esp_get_model_info(esp_ctx *ctx, int *out5);
+
+/* ---- replicas ---------------------------------------------------------------------- */
+/* R replicas (configurations), temperature slot t holds tems[t] (ascending for PT,
+ * parameters.F90:601).  spins[R][N][3]; replica r starts in slot r.  Resets the running
+ * energies to the direct energies, the swap map, and all counters. */
+int esp_set_replicas(esp_ctx *ctx, int R, const double *tems, const double *spins);
+/* spins[R][N][3] of each replica (configuration), or of each slot when by_slot != 0. */
+int esp_get_spins(esp_ctx *ctx, int by_slot, double *spins);
+/* Overwrite the spins (e.g. after a checkpoint read); resets the running energies. */
+int esp_put_spins(esp_ctx *ctx, const double *spins);
+
+/* ---- hot path -------------------------------------------------------------------- */
+/* energy_get (get_quant.F90:186-261) of every replica: E[R] (by replica). */
+int esp_energy(esp_ctx *ctx, double *E);
+/* nsweeps Metropolis sweeps of every replica at its slot's temperature, in colour order:
+ * metropolis + delta_energy_get (get_quant.F90:52-182, 264-337) with Philox4x32-10
+ * uniforms, counter (site, sweep, replica), key seed.  Sweep k uses counter sweep0+k. */
+int esp_sweep(esp_ctx *ctx, int nsweeps, int mode, double tilt, uint64_t seed, uint64_t sweep0);
+/* Deterministic mode: one sweep in colour order with proposals prop[R][N][3] (new spin
+ * vectors) and accept uniforms unif[R][N] given by the host; accepted[R][N] out (1/0). */
+int esp_sweep_hostfed(esp_ctx *ctx, const double *prop, const double *unif, uint8_t *accepted);
+/* One measurement (mcarlo.F90:536-543): accumulate E, E^2, E^4 of the running energy and
+ * |m|, m^2, m^4 (magnetization_get, get_quant.F90:340-363) into each replica's slot. */
+int esp_measure(esp_ctx *ctx);
+/* pt_swap (pt.F90:1235-1288) on the device in label form: pairs (t,t+1) for
+ * t = start-1, start+1, ... (start = 1 or 2, the reference's 1-based parity); uniform of
+ * pair t from Philox counter (t, step, 0xFFFFFFFF-stream), key seed^0x5bd1e995. */
+int esp_pt_exchange(esp_ctx *ctx, int start, uint64_t seed, uint64_t step);
+
+/* Fused block of the PT / MC schedule (pt.F90:654-688 and 773-908 without I/O):
+ * n_rounds x { steps_per_round x { steps_measure sweeps; [measure] }; [exchange] }.
+ * Exchange parity starts at start_parity (1 or 2) and alternates per round.
+ * Kernels are enqueued back-to-back (captured in a hipGraph when use_graph != 0). */
+int esp_run(esp_ctx *ctx, int n_rounds, int steps_per_round, int steps_measure,
+            int do_measure, int do_exchange, int start_parity,
+            int mode, double tilt, uint64_t seed, uint64_t sweep0, uint64_t step0,
+            int use_graph);
+
+/* Timing of the sweep kernels alone: nsweeps sweeps with a HIP event pair recorded on the
+ * context's stream around every sweep-kernel launch.  kernel_ms = sum of the per-launch
+ * elapsed times, nlaunch = launches, wall_ms = first start to last stop,
+ * updates_per_launch = attempted updates one launch performs (mean over colours). */
+int esp_time_sweeps(esp_ctx *ctx, int nsweeps, int mode, double tilt, uint64_t seed,
+                    uint64_t sweep0, double *kernel_ms, int *nlaunch, double *wall_ms,
+                    int64_t *updates_per_launch);
+
+/* ---- statistics ------------------------------------------------------------------ */
+/* acc[R][6] by slot: sums of E, E^2, E^4, |m|, m^2, m^4; rejected[R] by slot;
+ * e_run[R] by replica (running energy "by adding differences"); nmeas = measurements. */
+int esp_get_stats(esp_ctx *ctx, double *acc, int64_t *rejected, double *e_run, int64_t *nmeas);
+/* slot_rep[R]: 1-based replica id at slot t (the reference's map, pt.F90:1282-1284);
+ * nswap[R][2]: attempts, accepts per slot (num_swap(2,R)). */
+int esp_get_map(esp_ctx *ctx, int *slot_rep, int64_t *nswap);
+/* Zero acc, rejected, nswap and the measurement count (phase boundaries, pt.F90:650). */
+int esp_reset_stats(esp_ctx *ctx);
+/* Wait for the context's stream. */
+int esp_sync(esp_ctx *ctx);
+/* The context's HIP stream (hipStream_t) for callers that time or order work on it. */
+void *esp_stream(esp_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
