@@ -126,6 +126,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--config", default="pt_fcc16")
     ap.add_argument("--path", type=int, default=0, help="0 auto, 1 global-memory phases, 2 LDS")
+    ap.add_argument("--tps", type=int, default=0, help="lanes per site (0 automatic)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -146,16 +147,15 @@ def main():
     import numpy as np
     from espins_amd import lattice as Lt
     from espins_amd.engine import Engine
-    try:
-        import torch
-        has_torch = torch.cuda.is_available()
-    except Exception:
-        has_torch = False
+    # No torch here: its wheel carries its own HIP runtime, and a second runtime in the
+    # process keeps libespins_hip.so from opening the device.  The barrier is the engine's
+    # stream synchronisation (every kernel of the path runs on that one stream).
 
     R = len(tems)
     eng = Engine(0)
     eng.set_model(m)
     eng.set_path(args.path)
+    eng.set_tps(args.tps)
     eng.set_replicas(tems, Lt.random_spins(m.N, R, seed, m.mag))
     info = eng.model_info()
     log(f"model: {info}")
@@ -163,20 +163,20 @@ def main():
     graph = not args.no_graph
 
     def run(n, sweep0, step0):
-        eng.run(n, steps_per_round=1, steps_measure=1, measure=True, exchange=pt,
-                start_parity=1 + (step0 % 2), mode="random", seed=seed, sweep0=sweep0,
-                step0=step0, graph=graph)
+        if pt:    # one round = one sweep + measurement + exchange (pt_steps_swap = 1)
+            eng.run(n, steps_per_round=1, steps_measure=1, measure=True, exchange=True,
+                    start_parity=1 + (step0 % 2), mode="random", seed=seed, sweep0=sweep0,
+                    step0=step0, graph=graph)
+        else:     # independent temperatures: all n measured steps in one round
+            eng.run(1, steps_per_round=n, steps_measure=1, measure=True, exchange=False,
+                    mode="random", seed=seed, sweep0=sweep0, graph=False)
 
     run(args.warmup, 0, 0)
     eng.reset_stats()
     eng.sync()
-    if has_torch:
-        torch.cuda.synchronize()
     t0 = time.perf_counter()
     run(args.steps, args.warmup, args.warmup)
     eng.sync()
-    if has_torch:
-        torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     updates = m.N * R * args.steps
     value = updates / dt

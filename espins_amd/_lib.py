@@ -53,6 +53,7 @@ _SIGS = {
                            _i, _dp, _dp, _i, _dp, _dp]),
     "esp_set_supercell": (_i, [_p, _ip]),
     "esp_set_path": (_i, [_p, _i]),
+    "esp_set_tps": (_i, [_p, _i]),
     "esp_plan_model": (_i, [_i, _i, _dp, _i, _ip, _ip, _dp, _i, _ip, _ip, _dp, _i, _ip, _ip, _dp,
                             _ip, _ip, _ip, _ip]),
     "esp_get_coloring": (_i, [_p, _ip, _ip]),

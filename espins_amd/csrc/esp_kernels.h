@@ -1,0 +1,550 @@
+// esp_kernels.h -- sweep, measurement, exchange and energy kernels (gfx950, wave64).
+//
+// Device layout of one replica r (N sites, colour-major device order, stride Ns > N):
+//   x[Ns], y[Ns], z[Ns]  spin components (structure of arrays: a gather of a partner spin
+//                     is three 8-byte loads whose addresses are contiguous across a wave);
+//                     index N holds a zero spin that padded bond slots point at
+//   w[Ns]             sum of the accepted dE of this site since the last rebase, so the
+//                     running energy ("by adding differences", mcarlo.F90:748) is
+//                     e_base[r] + sum_i w_i without atomics and reproducibly
+// Bonds: nbr[zmax][N] device partner indices (self-padded); coefficients per (basis, slot)
+// in LDS when the lattice is periodic, else cJ/cB[zmax][N], cD[zmax][N][3].
+#pragma once
+#include "esp_device.h"
+
+namespace espk {
+using namespace espd;
+
+constexpr int kBlock = 256;      // phase / reduce kernels (4 waves)
+constexpr int kLdsBlock = 512;   // LDS-resident kernel (8 waves per replica)
+constexpr int kChunk = 8;        // partner gathers issued together per lane
+
+struct Clock {                   // device-side schedule counters (graph-replayable)
+  unsigned long long sweep;      // Philox sweep counter of the next sweep
+  unsigned long long step;       // exchange step counter
+  int parity;                    // next exchange start (1 or 2)
+  int pad;
+};
+
+struct KArgs {
+  double *sp;                    // [R][3][Ns]
+  double *w;                     // [R][Ns]
+  const int *nbr;
+  const double *cJ, *cB, *cD;
+  const double *ctab;            // [nb][zmax][5]
+  const double *mag;
+  const uint8_t *basis;
+  const int *dev2ref;
+  const int *col_off;
+  const double *si4, *h4;
+  const double *beta_slot;
+  const int *rep_slot;
+  unsigned long long *rej;
+  const Clock *clock;
+  const double *prop, *unif;     // host-fed mode, reference order
+  uint8_t *accepted;
+  int N, Ns, nb, zmax, R, ncol;
+  int c_begin, c_count, nblk;
+  int has_si, has_h, xcd_remap;
+  int sweep_off, tps;            // tps: lanes cooperating on one site (1, 2, 4, 8)
+  double tilt;
+  uint32_t key0, key1, stream_base;
+};
+
+enum { M_RANDOM = 0, M_CONST = 1, M_HOSTFED = 2 };
+
+// 32-bit byte offset from a wave-uniform base: lets the compiler use the SGPR-base
+// addressing form (one VGPR offset per lane, no 64-bit address arithmetic).  The host keeps
+// every table and replica below 4 GiB.
+template <class T>
+__device__ __forceinline__ T ld_off(const T *base, unsigned byte_off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+
+struct GSpins {                  // one replica in global memory
+  double *x, *y, *z;
+  __device__ __forceinline__ V3 ld(unsigned j) const {
+    const unsigned o = j * 8u;
+    return {ld_off(x, o), ld_off(y, o), ld_off(z, o)};
+  }
+};
+struct LSpins {                  // one replica in LDS (x, y, z, w)
+  double4 *p;
+  __device__ __forceinline__ V3 ld(unsigned j) const { const double4 t = p[j]; return {t.x, t.y, t.z}; }
+};
+
+struct Proposal {
+  V3 so, sn;
+  double u3;
+};
+
+// Proposal of one attempted update (identical on every lane of the site's group).
+template <int MODE, class SP>
+__device__ __forceinline__ Proposal propose(const KArgs &a, const SP &sp, int d, int r,
+                                            unsigned long long sweep) {
+  Proposal p;
+  p.so = sp.ld(d);
+  const int ir = a.dev2ref[d];
+  if (MODE == M_HOSTFED) {
+    const double *q = a.prop + 3 * ((size_t)r * a.N + ir);
+    p.sn = {q[0], q[1], q[2]};
+    p.u3 = a.unif[(size_t)r * a.N + ir];
+  } else {
+    const uint4 rn = philox10(make_uint4((uint32_t)ir, (uint32_t)sweep, (uint32_t)(sweep >> 32),
+                                         a.stream_base + (uint32_t)r),
+                              a.key0, a.key1);
+    const double mag = a.mag[d];
+    p.sn = MODE == M_RANDOM ? propose_random(mag, u01(rn.x), u01(rn.y))
+                            : propose_const(mag, p.so, a.tilt, u01(rn.x), u01(rn.y));
+    p.u3 = u01(rn.z);
+  }
+  return p;
+}
+
+// One bond's contribution to the J, B, D sums of delta_energy_get (get_quant.F90:288-336).
+template <bool CMP, bool HB, bool HD>
+__device__ __forceinline__ void bond_term(const KArgs &a, const double *crow, int l, int d,
+                                          const V3 &ds, const V3 &si, const V3 &sj, double &eJ,
+                                          double &eB, double &eD) {
+  double cj, cb = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+  if (CMP) {
+    cj = crow[5 * l];
+    if (HB) cb = crow[5 * l + 1];
+    if (HD) { cx = crow[5 * l + 2]; cy = crow[5 * l + 3]; cz = crow[5 * l + 4]; }
+  } else {
+    cj = a.cJ[(size_t)l * a.N + d];
+    if (HB) cb = a.cB[(size_t)l * a.N + d];
+    if (HD) {
+      const double *dv = a.cD + 3 * ((size_t)l * a.N + d);
+      cx = dv[0]; cy = dv[1]; cz = dv[2];
+    }
+  }
+  const double dsj = dot(ds, sj);
+  eJ = eJ - cj * dsj;
+  if (HB) {
+    const double sij = dot(si, sj);
+    eB = eB + cb * (dsj * dsj + 2.0 * sij * dsj);
+  }
+  if (HD)  // D . (dS x S_j)  (utility_cross, utility.F90:212-234)
+    eD = eD + (cx * (ds.y * sj.z - ds.z * sj.y) + cy * (ds.z * sj.x - ds.x * sj.z) +
+               cz * (ds.x * sj.y - ds.y * sj.x));
+}
+
+// Partial bond sums over the slots l = g, g+tps, ... of site d.  Inactive lanes return
+// zeros.  Full chunks issue kChunk index loads, then kChunk partner gathers, then the
+// arithmetic, with no masks; only the last partial chunk is masked.  Padded slots point at
+// the zero spin (index N), so they need no test; a self bond (SELF) reads the new spin.
+template <bool CMP, bool HB, bool HD, bool SELF, class SP>
+__device__ __forceinline__ void bond_sums(const KArgs &a, const SP &sp, const double *ctab, int d,
+                                          bool active, int g, const Proposal &p, double &eJ,
+                                          double &eB, double &eD) {
+  eJ = 0.0; eB = 0.0; eD = 0.0;
+  if (!active) return;
+  const V3 ds = sub(p.sn, p.so);
+  const V3 si = sub(p.sn, ds);  // old spin as the reference recovers it (get_quant.F90:306,320)
+  const int tps = a.tps;
+  const int nk = (a.zmax - g + tps - 1) / tps;
+  const int nfull = nk - nk % kChunk;
+  const double *crow = CMP ? ctab + (size_t)a.basis[d] * a.zmax * 5 : nullptr;
+  const unsigned ud = (unsigned)d;
+  for (int k0 = 0; k0 < nfull; k0 += kChunk) {
+    unsigned jj[kChunk];
+    V3 sv[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u)
+      jj[u] = (unsigned)ld_off(a.nbr, ((unsigned)(g + tps * (k0 + u)) * (unsigned)a.N + ud) * 4u);
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+      const V3 t = sp.ld(jj[u]);
+      sv[u] = (SELF && jj[u] == ud) ? p.sn : t;
+    }
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) bond_term<CMP, HB, HD>(a, crow, g + tps * (k0 + u), d, ds, si, sv[u], eJ, eB, eD);
+  }
+  const int rem = nk - nfull;
+  if (rem > 0) {
+    unsigned jj[kChunk];
+    V3 sv[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+      const int l = u < rem ? g + tps * (nfull + u) : g;
+      jj[u] = (unsigned)ld_off(a.nbr, ((unsigned)l * (unsigned)a.N + ud) * 4u);
+    }
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+      const V3 t = sp.ld(jj[u]);
+      sv[u] = (SELF && jj[u] == ud) ? p.sn : t;
+    }
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u)
+      if (u < rem) bond_term<CMP, HB, HD>(a, crow, g + tps * (nfull + u), d, ds, si, sv[u], eJ, eB, eD);
+  }
+}
+
+// Sum over the tps lanes of a site's group: xor butterfly, bitwise identical on every lane.
+__device__ __forceinline__ double group_sum(double v, int tps) {
+  for (int o = 1; o < tps; o <<= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Local terms and the accept test of metropolis (get_quant.F90:166-178): reject iff
+// dE > 0 and exp(-beta dE) < u3.  Returns dE through *de.
+__device__ __forceinline__ bool decide(const KArgs &a, int d, const Proposal &p, double eJ,
+                                       double eB, double eD, double beta, double *de_out) {
+  const V3 ds = sub(p.sn, p.so);
+  const V3 si = sub(p.sn, ds);
+  const int b = a.basis[d];
+  double de = eJ;
+  if (a.has_h) {
+    const double *h = a.h4 + 4 * b;
+    de = de + h[0] * (ds.x * h[1] + ds.y * h[2] + ds.z * h[3]);
+  }
+  if (a.has_si) {
+    const double *k = a.si4 + 4 * b;
+    const double sa = si.x * k[1] + si.y * k[2] + si.z * k[3];
+    const double dsa = ds.x * k[1] + ds.y * k[2] + ds.z * k[3];
+    de = de + k[0] * (dsa * dsa + 2.0 * sa * dsa);
+  }
+  de = de + eB;
+  de = de + eD;
+  *de_out = de;
+  return (de > 0.0) && (exp(-beta * de) < p.u3);
+}
+
+__device__ __forceinline__ void decode_block(const KArgs &a, int &r, int &b) {
+  const int L = blockIdx.x;
+  if (a.xcd_remap) {  // blocks go round-robin over the 8 XCDs: keep a replica on one XCD/L2
+    const int x = L & 7, q = L >> 3;
+    r = x + 8 * (q / a.nblk);
+    b = q % a.nblk;
+  } else {
+    r = L / a.nblk;
+    b = L % a.nblk;
+  }
+}
+
+// One colour phase over every replica (global-memory path).  a.tps lanes per site.
+template <int MODE, bool CMP, bool HB, bool HD, bool SELF>
+__global__ __launch_bounds__(kBlock) void k_phase(KArgs a) {
+  extern __shared__ double s_ctab[];
+  int r, b;
+  decode_block(a, r, b);
+  if (r >= a.R) return;  // uniform per block
+  if (CMP) {
+    const int n = a.nb * a.zmax * 5;
+    for (int k = threadIdx.x; k < n; k += kBlock) s_ctab[k] = a.ctab[k];
+    __syncthreads();
+  }
+  const int tps = a.tps;
+  const int g = threadIdx.x & (tps - 1);
+  const int k = (b * kBlock + threadIdx.x) / tps;
+  const bool active = k < a.c_count;
+  const int d = a.c_begin + (active ? k : 0);
+  const int slot = a.rep_slot[r];
+  const double beta = a.beta_slot[slot];
+  const unsigned long long sweep = a.clock->sweep + (unsigned long long)a.sweep_off;
+  double *base = a.sp + (size_t)r * 3 * a.Ns;
+  const GSpins sp{base, base + a.Ns, base + 2 * a.Ns};
+  Proposal p{};
+  if (active) p = propose<MODE>(a, sp, d, r, sweep);
+  double eJ, eB, eD;
+  bond_sums<CMP, HB, HD, SELF>(a, sp, s_ctab, d, active, g, p, eJ, eB, eD);
+  eJ = group_sum(eJ, tps);
+  if (HB) eB = group_sum(eB, tps);
+  if (HD) eD = group_sum(eD, tps);
+  bool rej = false;
+  if (active) {
+    double de;
+    rej = decide(a, d, p, eJ, eB, eD, beta, &de);
+    if (g == 0) {
+      if (MODE == M_HOSTFED) a.accepted[(size_t)r * a.N + a.dev2ref[d]] = rej ? 0 : 1;
+      if (!rej) {
+        sp.x[d] = p.sn.x; sp.y[d] = p.sn.y; sp.z[d] = p.sn.z;
+        a.w[(size_t)r * a.Ns + d] += de;
+      }
+    }
+  }
+  const unsigned long long m = __ballot(rej && g == 0);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(&a.rej[slot], (unsigned long long)__popcll(m));
+}
+
+struct LdsMeas {
+  double *acc;                   // [R][6] by slot
+  double *e_run;                 // [R] by replica
+  const double *e_base;
+  unsigned long long *nmeas;
+  int measure_every;             // 0 = never
+};
+
+// All colours of nsweeps sweeps of one replica per block, spins resident in LDS,
+// measurement (mcarlo.F90:536-543) fused every measure_every sweeps.
+template <int MODE, bool CMP, bool HB, bool HD, bool SELF>
+__global__ __launch_bounds__(kLdsBlock) void k_lds(KArgs a, int nsweeps, LdsMeas m) {
+  extern __shared__ double4 s_sp[];
+  __shared__ double red[4 * (kLdsBlock / 64)];
+  const int r = blockIdx.x;
+  double *ctab = reinterpret_cast<double *>(s_sp + a.N + 1);
+  const double *gx = a.sp + (size_t)r * 3 * a.Ns;
+  double *gw = a.w + (size_t)r * a.Ns;
+  for (int d = threadIdx.x; d <= a.N; d += kLdsBlock)   // index N: the zero spin
+    s_sp[d] = d < a.N ? make_double4(gx[d], gx[a.Ns + d], gx[2 * a.Ns + d], gw[d])
+                      : make_double4(0.0, 0.0, 0.0, 0.0);
+  if (CMP)
+    for (int k = threadIdx.x; k < a.nb * a.zmax * 5; k += kLdsBlock) ctab[k] = a.ctab[k];
+  __syncthreads();
+  const LSpins sp{s_sp};
+  const int slot = a.rep_slot[r];
+  const double beta = a.beta_slot[slot];
+  const unsigned long long sweep0 = a.clock->sweep;
+  const int tps = a.tps;
+  const int g = threadIdx.x & (tps - 1);
+  const int per = kLdsBlock / tps;
+  unsigned int nrej = 0;
+  for (int sw = 0; sw < nsweeps; ++sw) {
+    const unsigned long long sweep = sweep0 + (unsigned long long)sw;
+    for (int c = 0; c < a.ncol; ++c) {
+      const int beg = a.col_off[c], end = a.col_off[c + 1];
+      for (int base = beg; base < end; base += per) {  // uniform trip count in the block
+        const int d0 = base + threadIdx.x / tps;
+        const bool active = d0 < end;
+        const int d = active ? d0 : beg;
+        Proposal p{};
+        if (active) p = propose<MODE>(a, sp, d, r, sweep);
+        double eJ, eB, eD;
+        bond_sums<CMP, HB, HD, SELF>(a, sp, ctab, d, active, g, p, eJ, eB, eD);
+        eJ = group_sum(eJ, tps);
+        if (HB) eB = group_sum(eB, tps);
+        if (HD) eD = group_sum(eD, tps);
+        if (active) {
+          double de;
+          const bool rej = decide(a, d, p, eJ, eB, eD, beta, &de);
+          if (g == 0) {
+            if (rej) ++nrej;
+            else {
+              const double4 old = s_sp[d];
+              s_sp[d] = make_double4(p.sn.x, p.sn.y, p.sn.z, old.w + de);
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+    const bool last = sw + 1 == nsweeps;
+    const bool meas = m.measure_every > 0 && (sw + 1) % m.measure_every == 0;
+    if (meas || last) {
+      double v[4] = {0, 0, 0, 0};
+      for (int d = threadIdx.x; d < a.N; d += kLdsBlock) {
+        const double4 s = s_sp[d];
+        v[0] += s.w; v[1] += s.x; v[2] += s.y; v[3] += s.z;
+      }
+      block_sum<4>(v, red);
+      if (threadIdx.x == 0) {
+        const double e = m.e_base[r] + v[0];
+        if (meas) {
+          const double n = (double)a.N;
+          const double mx = v[1] / n, my = v[2] / n, mz = v[3] / n;
+          const double m2 = mx * mx + my * my + mz * mz;
+          double *acc = m.acc + 6 * slot;
+          const double e2 = e * e;
+          acc[0] += e; acc[1] += e2; acc[2] += e2 * e2;
+          acc[3] += sqrt(m2); acc[4] += m2; acc[5] += m2 * m2;
+          if (r == 0) m.nmeas[0] += 1;
+        }
+        if (last) m.e_run[r] = e;
+      }
+    }
+  }
+  double *ox = a.sp + (size_t)r * 3 * a.Ns;
+  for (int d = threadIdx.x; d < a.N; d += kLdsBlock) {
+    const double4 s = s_sp[d];
+    ox[d] = s.x; ox[a.Ns + d] = s.y; ox[2 * a.Ns + d] = s.z; gw[d] = s.w;
+  }
+  unsigned int v = nrej;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(&a.rej[slot], (unsigned long long)v);
+}
+
+// Partial sums of (w, x, y, z) per (replica, block) for the running energy and m.
+__global__ __launch_bounds__(kBlock) void k_reduce(const double *sp, const double *w, int N, int Ns,
+                                                   int nblk, double *partial) {
+  __shared__ double red[4 * (kBlock / 64)];
+  const int r = blockIdx.y, b = blockIdx.x;
+  const int chunk = (N + nblk - 1) / nblk;
+  const int beg = b * chunk, end = min(N, beg + chunk);
+  double v[4] = {0, 0, 0, 0};
+  const double *x = sp + (size_t)r * 3 * Ns;
+  const double *wr = w + (size_t)r * Ns;
+  for (int d = beg + threadIdx.x; d < end; d += kBlock) {
+    v[0] += wr[d]; v[1] += x[d]; v[2] += x[Ns + d]; v[3] += x[2 * Ns + d];
+  }
+  block_sum<4>(v, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 4; ++k) partial[((size_t)r * nblk + b) * 4 + k] = v[k];
+}
+
+struct FinArgs {
+  const double *partial;         // [R][nblk][4] or null (e_run already fresh)
+  int nblk, N, R;
+  const double *e_base;
+  double *e_run;
+  double *acc;
+  unsigned long long *nmeas;
+  int *slot_rep, *rep_slot;
+  const double *beta_slot;
+  unsigned long long *nswap;     // [R][2]
+  Clock *clock;
+  int measure, exchange, advance_sweeps;
+  uint32_t key0, key1, stream_base;
+};
+
+// Running energies / measurement, then the replica exchange (pt_swap, pt.F90:1235-1288)
+// in label form, then the schedule clock.  One block.
+__global__ __launch_bounds__(1024) void k_finalize(FinArgs f) {
+  if (f.partial)
+    for (int r = threadIdx.x; r < f.R; r += blockDim.x) {
+      double v[4] = {0, 0, 0, 0};
+      const double *p = f.partial + (size_t)r * f.nblk * 4;
+      for (int b = 0; b < f.nblk; ++b)
+        for (int k = 0; k < 4; ++k) v[k] += p[4 * b + k];
+      const double e = f.e_base[r] + v[0];
+      f.e_run[r] = e;
+      if (f.measure) {
+        const double n = (double)f.N;
+        const double mx = v[1] / n, my = v[2] / n, mz = v[3] / n;
+        const double m2 = mx * mx + my * my + mz * mz;
+        double *acc = f.acc + 6 * f.rep_slot[r];
+        const double e2 = e * e;
+        acc[0] += e; acc[1] += e2; acc[2] += e2 * e2;
+        acc[3] += sqrt(m2); acc[4] += m2; acc[5] += m2 * m2;
+      }
+    }
+  __syncthreads();
+  if (f.exchange) {
+    const int start = f.clock->parity;  // 1 or 2 (pt.F90:679,898)
+    const unsigned long long step = f.clock->step;
+    for (int t = start - 1 + 2 * threadIdx.x; t + 1 < f.R; t += 2 * blockDim.x) {
+      const int ra = f.slot_rep[t], rb = f.slot_rep[t + 1];
+      const double fac = (f.beta_slot[t] - f.beta_slot[t + 1]) * (f.e_run[ra] - f.e_run[rb]);
+      bool sw = fac > 0.0;
+      if (!sw) {
+        const uint4 rn = philox10(make_uint4((uint32_t)t, (uint32_t)step, (uint32_t)(step >> 32),
+                                             0xFFFFFFFFu - f.stream_base),
+                                  f.key0 ^ 0x5bd1e995u, f.key1);
+        sw = u01(rn.x) < exp(fac);
+      }
+      f.nswap[2 * t] += 1;
+      f.nswap[2 * (t + 1)] += 1;
+      if (sw) {
+        f.slot_rep[t] = rb; f.slot_rep[t + 1] = ra;
+        f.rep_slot[rb] = t; f.rep_slot[ra] = t + 1;
+        f.nswap[2 * t + 1] += 1;
+        f.nswap[2 * (t + 1) + 1] += 1;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (f.measure) f.nmeas[0] += 1;
+    if (f.exchange) {
+      f.clock->step += 1;
+      f.clock->parity = 3 - f.clock->parity;
+    }
+    f.clock->sweep += (unsigned long long)f.advance_sweeps;
+  }
+}
+
+// Direct energy (energy_get, get_quant.F90:186-261): per-site terms, block partials.
+template <bool CMP, bool HB, bool HD>
+__global__ __launch_bounds__(kBlock) void k_energy(KArgs a, int nblk, double *partial) {
+  extern __shared__ double s_ctab[];
+  __shared__ double red[kBlock / 64];
+  const int r = blockIdx.y, bk = blockIdx.x;
+  if (CMP) {
+    for (int k = threadIdx.x; k < a.nb * a.zmax * 5; k += kBlock) s_ctab[k] = a.ctab[k];
+    __syncthreads();
+  }
+  const int chunk = (a.N + nblk - 1) / nblk;
+  const int beg = bk * chunk, end = min(a.N, beg + chunk);
+  double *base = a.sp + (size_t)r * 3 * a.Ns;
+  const GSpins sp{base, base + a.Ns, base + 2 * a.Ns};
+  double v[1] = {0.0};
+  for (int d = beg + threadIdx.x; d < end; d += kBlock) {
+    const V3 s = sp.ld(d);
+    const int b = a.basis[d];
+    double e = 0.0;
+    for (int l = 0; l < a.zmax; ++l) {
+      const unsigned j = (unsigned)a.nbr[(size_t)l * a.N + d];
+      const V3 t = sp.ld(j);
+      const double dd = dot(s, t);
+      double cj, cb = 0, cx = 0, cy = 0, cz = 0;
+      if (CMP) {
+        const double *c = s_ctab + ((size_t)b * a.zmax + l) * 5;
+        cj = c[0]; cb = c[1]; cx = c[2]; cy = c[3]; cz = c[4];
+      } else {
+        cj = a.cJ[(size_t)l * a.N + d];
+        if (HB) cb = a.cB[(size_t)l * a.N + d];
+        if (HD) {
+          const double *dv = a.cD + 3 * ((size_t)l * a.N + d);
+          cx = dv[0]; cy = dv[1]; cz = dv[2];
+        }
+      }
+      e = e - cj * dd;
+      if (HB) e = e + cb * dd * dd;
+      if (HD)
+        e = e + (cx * (s.y * t.z - s.z * t.y) + cy * (s.z * t.x - s.x * t.z) +
+                 cz * (s.x * t.y - s.y * t.x));
+    }
+    if (a.has_h) {
+      const double *h = a.h4 + 4 * b;
+      e = e + 2.0 * h[0] * (s.x * h[1] + s.y * h[2] + s.z * h[3]);
+    }
+    if (a.has_si) {
+      const double *k = a.si4 + 4 * b;
+      const double sa = s.x * k[1] + s.y * k[2] + s.z * k[3];
+      e = e + 2.0 * k[0] * sa * sa;
+    }
+    v[0] += e;
+  }
+  block_sum<1>(v, red);
+  if (threadIdx.x == 0) partial[(size_t)r * nblk + bk] = v[0];
+}
+
+__global__ void k_energy_fin(const double *partial, int nblk, int R, double *E) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += partial[(size_t)r * nblk + b];
+  E[r] = s / 2.0;
+}
+
+// Spins in: reference order [R][N][3] -> device order SoA, w = 0.
+__global__ void k_put_spins(const double *in, const int *ref2dev, int N, int Ns, int R, double *sp,
+                            double *w) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)N * R) return;
+  const int r = (int)(t / N), i = (int)(t % N);
+  const double *p = in + 3 * t;
+  const int d = ref2dev[i];
+  double *x = sp + (size_t)r * 3 * Ns;
+  x[d] = p[0]; x[Ns + d] = p[1]; x[2 * Ns + d] = p[2];
+  w[(size_t)r * Ns + d] = 0.0;
+}
+
+__global__ void k_get_spins(const double *sp, const int *ref2dev, const int *src_rep, int N, int Ns,
+                            int R, double *out) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)N * R) return;
+  const int q = (int)(t / N), i = (int)(t % N);
+  const int r = src_rep ? src_rep[q] : q;
+  const double *x = sp + (size_t)r * 3 * Ns;
+  const int d = ref2dev[i];
+  double *p = out + 3 * t;
+  p[0] = x[d]; p[1] = x[Ns + d]; p[2] = x[2 * Ns + d];
+}
+
+__global__ void k_fill(double *p, size_t n, double v) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+}  // namespace espk

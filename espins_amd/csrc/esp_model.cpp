@@ -436,6 +436,7 @@ std::string prepare_model(HostModel &hm, int N, int nb, const double *mag, int z
 
   const int zp = std::max(zmax, 1);
   hm.zmax = zp;
+  hm.has_self = false;
   hm.nbr.assign((size_t)zp * N, 0);
   if (!comp) hm.coef.assign((size_t)zp * N * kCoef, 0.0);
   else hm.ctab.assign((size_t)nb * zp * kCoef, 0.0);
@@ -447,7 +448,9 @@ std::string prepare_model(HostModel &hm, int N, int nb, const double *mag, int z
     hm.basis[d] = (uint8_t)(i % nb);
     for (int l = 0; l < zp; ++l) {
       const bool real = l < (int)bonds[i].size();
-      hm.nbr[(size_t)l * N + d] = real ? hm.ref2dev[bonds[i][l].j] : d;  // self-padded
+      // padding slots point at the all-zero spin kept at device index N
+      hm.nbr[(size_t)l * N + d] = real ? hm.ref2dev[bonds[i][l].j] : N;
+      if (real && bonds[i][l].j == i) hm.has_self = true;
       if (!comp && real)
         for (int k = 0; k < kCoef; ++k) hm.coef[((size_t)l * N + d) * kCoef + k] = bonds[i][l].c[k];
     }

@@ -22,10 +22,11 @@ struct HostModel {
   int ncol = 0;
   bool has_B = false, has_D = false, has_si = false, has_h = false;
   bool compressed = false;      // coefficients depend only on (basis, slot)
+  bool has_self = false;        // some bond joins a site to itself (1-cell supercell axis)
   std::vector<int> colour;      // [N] by reference site (0-based)
   std::vector<int> col_off;     // [ncol+1] device ranges
   std::vector<int> dev2ref, ref2dev;
-  std::vector<int> nbr;         // [zmax][N] device neighbour index (self-padded)
+  std::vector<int> nbr;         // [zmax][N] device partner index (N = the zero spin)
   std::vector<double> coef;     // [zmax][N][kCoef] when !compressed
   std::vector<double> ctab;     // [nb][zmax][kCoef] when compressed
   std::vector<double> mag;      // [N] device order

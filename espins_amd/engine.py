@@ -106,6 +106,10 @@ class Engine:
         """0 auto, 1 global-memory colour phases, 2 LDS-resident."""
         self._chk(E.lib.esp_set_path(self._h, int(path)))
 
+    def set_tps(self, tps):
+        """Lanes per site in the sweep kernels (1, 2, 4, 8; 0 = automatic)."""
+        self._chk(E.lib.esp_set_tps(self._h, int(tps)))
+
     def coloring(self):
         n = C.c_int()
         col = np.zeros(self.N, np.int32)
@@ -113,10 +117,10 @@ class Engine:
         return n.value, col
 
     def model_info(self):
-        out = np.zeros(5, np.int32)
+        out = np.zeros(6, np.int32)
         self._chk(E.lib.esp_get_model_info(self._h, E.ptr(out, C.c_int)))
         return dict(N=int(out[0]), zmax=int(out[1]), ncol=int(out[2]), compressed=bool(out[3]),
-                    lds=bool(out[4]))
+                    lds=bool(out[4]), tps=int(out[5]))
 
     # ---- replicas --------------------------------------------------------------------
     def set_replicas(self, tems, spins):

@@ -88,8 +88,12 @@ int esp_set_path(esp_ctx *ctx, int path);
 /* colour[N]: 0-based colour of 1-based site i at colour[i-1]; *ncol colours. */
 int esp_get_coloring(esp_ctx *ctx, int *ncol, int *colour);
 /* Model facts: out[0]=N, out[1]=merged bond width, out[2]=colours, out[3]=1 if the
- * couplings were compressed to per-basis tables, out[4]=1 if the LDS-resident sweep is used. */
-int esp_get_model_info(esp_ctx *ctx, int *out5);
+ * couplings were compressed to per-basis tables, out[4]=1 if the LDS-resident sweep is used,
+ * out[5]=lanes per site in the sweep kernels. */
+int esp_get_model_info(esp_ctx *ctx, int *out6);
+/* Lanes cooperating on one site's bond sum in the sweep kernels: 1, 2, 4 or 8; 0 = automatic
+ * (from the bond width). */
+int esp_set_tps(esp_ctx *ctx, int tps);
 
 /* ---- replicas ---------------------------------------------------------------------- */
 /* R replicas (configurations), temperature slot t holds tems[t] (ascending for PT,
