@@ -74,7 +74,7 @@ def build_driver(verbose: bool = False):
         return None
     if not (shutil.which("amdflang") or os.path.exists("/opt/rocm/bin/amdflang")):
         return None
-    _run(["make", "-C", ddir])
+    _run(["make", "-C", ddir, "-B"])
     return os.path.join(ddir, "mc_gpu.x")
 
 

@@ -1,0 +1,138 @@
+! espins_hip.f90 -- ISO_C_BINDING interface to libespins_hip.so (include/espins_hip.h).
+!
+! Every entry point returns an integer(c_int) status: ESP_OK = 0, negative on error, with the
+! message from esp_last_error.  Arrays are passed as the reference's own Fortran arrays:
+! spin_matrix(3,N,R), jij_matrix(N,z), jij_nbors_matrix(N,z) (1-based), dij_matrix(3,N,z).
+module espins_hip
+  use iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: ESP_OK = 0, ESP_MODE_RANDOM = 0, ESP_MODE_CONST = 1
+
+  public :: esp_create, esp_destroy, esp_last_error, esp_version, esp_build_table, &
+            esp_set_supercell, esp_set_model, esp_get_model_info, esp_set_replicas, &
+            esp_get_spins, esp_put_spins, esp_energy, esp_run, esp_get_stats, esp_get_map, &
+            esp_reset_stats, esp_sync, esp_error_text
+
+  interface
+    integer(c_int) function esp_create(ctx, device) bind(C, name='esp_create')
+      import :: c_int, c_ptr
+      type(c_ptr), intent(out) :: ctx
+      integer(c_int), value :: device
+    end function
+    integer(c_int) function esp_destroy(ctx) bind(C, name='esp_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    type(c_ptr) function esp_last_error(ctx) bind(C, name='esp_last_error')
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function esp_version() bind(C, name='esp_version')
+      import :: c_int
+    end function
+    integer(c_int) function esp_build_table(nb, basis_frac, basis_mag, L, open_boundary, nrows, &
+        f1, f2, value, dvec, ncomp, zcap, cnt, nbr, val, zmax) bind(C, name='esp_build_table')
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: nb, open_boundary, nrows, ncomp, zcap
+      real(c_double), intent(in) :: basis_frac(*), basis_mag(*), f1(*), f2(*), value(*)
+      type(c_ptr), value :: dvec
+      integer(c_int), intent(in) :: L(3)
+      integer(c_int), intent(out) :: cnt(*), nbr(*), zmax
+      real(c_double), intent(out) :: val(*)
+    end function
+    integer(c_int) function esp_set_supercell(ctx, L) bind(C, name='esp_set_supercell')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(in) :: L(3)
+    end function
+    integer(c_int) function esp_set_model(ctx, N, nb, mag, zJ, j_cnt, j_nbr, j_val, &
+        zB, b_cnt, b_nbr, b_val, zD, d_cnt, d_nbr, d_vec, have_si, si_par, si_axis, &
+        have_field, h_par, h_axis) bind(C, name='esp_set_model')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: N, nb, zJ, zB, zD, have_si, have_field
+      real(c_double), intent(in) :: mag(*)
+      integer(c_int), intent(in) :: j_cnt(*), j_nbr(*)
+      real(c_double), intent(in) :: j_val(*)
+      type(c_ptr), value :: b_cnt, b_nbr, b_val, d_cnt, d_nbr, d_vec, si_par, si_axis, h_par, h_axis
+    end function
+    integer(c_int) function esp_get_model_info(ctx, out6) bind(C, name='esp_get_model_info')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: out6(6)
+    end function
+    integer(c_int) function esp_set_replicas(ctx, R, tems, spins) bind(C, name='esp_set_replicas')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: R
+      real(c_double), intent(in) :: tems(*), spins(*)
+    end function
+    integer(c_int) function esp_get_spins(ctx, by_slot, spins) bind(C, name='esp_get_spins')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: by_slot
+      real(c_double), intent(out) :: spins(*)
+    end function
+    integer(c_int) function esp_put_spins(ctx, spins) bind(C, name='esp_put_spins')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: spins(*)
+    end function
+    integer(c_int) function esp_energy(ctx, E) bind(C, name='esp_energy')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: E(*)
+    end function
+    integer(c_int) function esp_run(ctx, n_rounds, steps_per_round, steps_measure, do_measure, &
+        do_exchange, start_parity, mode, tilt, seed, sweep0, step0, use_graph) bind(C, name='esp_run')
+      import :: c_int, c_double, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n_rounds, steps_per_round, steps_measure, do_measure, do_exchange, &
+                               start_parity, mode, use_graph
+      real(c_double), value :: tilt
+      integer(c_int64_t), value :: seed, sweep0, step0
+    end function
+    integer(c_int) function esp_get_stats(ctx, acc, rejected, e_run, nmeas) bind(C, name='esp_get_stats')
+      import :: c_int, c_double, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: acc(*), e_run(*)
+      integer(c_int64_t), intent(out) :: rejected(*), nmeas
+    end function
+    integer(c_int) function esp_get_map(ctx, slot_rep, nswap) bind(C, name='esp_get_map')
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: slot_rep(*)
+      integer(c_int64_t), intent(out) :: nswap(*)
+    end function
+    integer(c_int) function esp_reset_stats(ctx) bind(C, name='esp_reset_stats')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function esp_sync(ctx) bind(C, name='esp_sync')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+  end interface
+
+contains
+
+  ! The library's message for the last failed call on ctx.
+  function esp_error_text(ctx) result(txt)
+    type(c_ptr), value :: ctx
+    character(len=512) :: txt
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: s(:)
+    integer :: k
+    txt = ' '
+    p = esp_last_error(ctx)
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, s, [512])
+    do k = 1, 512
+      if (s(k) == c_null_char) exit
+      txt(k:k) = s(k)
+    end do
+  end function
+
+end module espins_hip

@@ -1,0 +1,502 @@
+! mc_input.f90 -- reader for the subset of the ESpinS .mcin format the hot path needs.
+!
+! Line syntax (parameters.F90:1724-1805): case-insensitive "key = v", "key : v" or "key v";
+! '!' and '#' start comments; "begin X ... end X" blocks with an optional unit line.
+! Atoms are sorted by species in order of first appearance (parameters.F90:2470-2540).
+! Coupling rows "f1=x,y,z:f2=x,y,z:jij=v" or c1=/c2= Cartesian (parameters.F90:2558-2846).
+module mc_input
+  use iso_c_binding, only: c_double, c_int
+  implicit none
+  private
+
+  integer, parameter, public :: dp = c_double
+  real(dp), parameter, public :: k_B = 8.6173303e-5_dp          ! constants.F90:48
+  real(dp), parameter, public :: hart = 27.21138602_dp          ! constants.F90:49
+  real(dp), parameter, public :: bohr_magn = 5.7883818012e-5_dp ! constants.F90:50
+  real(dp), parameter, public :: bohr = 0.52917721092_dp
+  integer, parameter :: maxl = 512
+
+  type, public :: rows_t
+    integer :: n = 0
+    real(dp), allocatable :: f1(:,:), f2(:,:), val(:), dvec(:,:)
+  end type
+
+  type, public :: input_t
+    real(dp) :: lat(3,3)                        ! lat(:,k) = lattice vector k (Angstrom)
+    integer :: nb = 0
+    character(len=8), allocatable :: label(:)
+    real(dp), allocatable :: basis(:,:), moment(:)
+    integer :: L(3) = 4
+    type(rows_t) :: J, B, D
+    logical :: have_B = .false., have_D = .false., have_si = .false., have_h = .false.
+    real(dp), allocatable :: si_par(:), si_axis(:,:), h_par(:), h_axis(:,:)
+    character(len=16) :: tems_mode = 'lin', mcarlo_mode = 'random', initial = 'ferro'
+    integer :: tems_num = 1, steps_warmup = 100000, steps_mc = 200000, steps_measure = 2
+    real(dp) :: tem_start = 5.0_dp, tem_end = 5.0_dp, tilt = 0.125_dp
+    real(dp), allocatable :: tems(:)
+    logical :: pt = .false.
+    integer :: pt_steps_swap = 1, pt_print_swap = 10000
+    integer :: seed = 0
+    logical :: have_seed = .false., open_boundary = .false.
+  end type
+
+  public :: read_input, input_error
+
+  character(len=maxl), allocatable :: lines(:)
+  integer :: nlines
+
+contains
+
+  subroutine input_error(msg)
+    character(len=*), intent(in) :: msg
+    write(*, '(1x,a)') 'Error: '//trim(msg)
+    error stop 1
+  end subroutine
+
+  pure function lower(s) result(t)
+    character(len=*), intent(in) :: s
+    character(len=len(s)) :: t
+    integer :: k, c
+    t = s
+    do k = 1, len(s)
+      c = iachar(s(k:k))
+      if (c >= 65 .and. c <= 90) t(k:k) = achar(c + 32)
+    end do
+  end function
+
+  subroutine load(fname)
+    character(len=*), intent(in) :: fname
+    integer :: u, ios, k, p
+    character(len=maxl) :: buf
+    open(newunit=u, file=fname, status='old', action='read', iostat=ios)
+    if (ios /= 0) call input_error('cannot open '//trim(fname))
+    nlines = 0
+    do
+      read(u, '(a)', iostat=ios) buf
+      if (ios /= 0) exit
+      nlines = nlines + 1
+    end do
+    allocate(lines(nlines))
+    rewind(u)
+    do k = 1, nlines
+      read(u, '(a)') buf
+      p = scan(buf, '!#')
+      if (p > 0) buf(p:) = ' '
+      do p = 1, len(buf)
+        if (iachar(buf(p:p)) == 9) buf(p:p) = ' '
+      end do
+      lines(k) = adjustl(lower(buf))
+    end do
+    close(u)
+  end subroutine
+
+  ! value string of keyword key, or found = .false.
+  subroutine keyword(key, found, val)
+    character(len=*), intent(in) :: key
+    logical, intent(out) :: found
+    character(len=*), intent(out) :: val
+    integer :: k, n
+    character(len=maxl) :: rest
+    found = .false.
+    val = ' '
+    n = len_trim(key)
+    do k = 1, nlines
+      if (lines(k)(1:n) /= key(1:n)) cycle
+      rest = lines(k)(n+1:)
+      if (len_trim(rest) > 0 .and. index(' =:', rest(1:1)) == 0) cycle
+      rest = adjustl(rest)
+      if (rest(1:1) == '=' .or. rest(1:1) == ':') rest = adjustl(rest(2:))
+      if (found) call input_error('keyword '//trim(key)//' found more than once')
+      found = .true.
+      val = rest
+    end do
+  end subroutine
+
+  logical function is_true(v)
+    character(len=*), intent(in) :: v
+    character(len=maxl) :: w
+    w = adjustl(v)
+    if (w(1:1) == '.') w = w(2:)
+    is_true = w(1:1) == 't'
+  end function
+
+  ! [first, last] body lines of block name (units line included), or found = .false.
+  subroutine block(name, found, first, last)
+    character(len=*), intent(in) :: name
+    logical, intent(out) :: found
+    integer, intent(out) :: first, last
+    integer :: k
+    character(len=maxl) :: w
+    found = .false.
+    first = 0
+    last = -1
+    do k = 1, nlines
+      if (lines(k)(1:5) /= 'begin') cycle
+      w = adjustl(lines(k)(6:))
+      if (trim(w) /= trim(name)) cycle
+      if (found) call input_error('block '//trim(name)//' found more than once')
+      found = .true.
+      first = k + 1
+    end do
+    if (.not. found) return
+    do k = first, nlines
+      if (lines(k)(1:3) == 'end') then
+        last = k - 1
+        return
+      end if
+    end do
+    call input_error('begin '//trim(name)//' without end')
+  end subroutine
+
+  subroutine cart_to_frac(lat, c, f)
+    real(dp), intent(in) :: lat(3,3), c(3)
+    real(dp), intent(out) :: f(3)
+    real(dp) :: a(3,3), det, inv(3,3)
+    a = lat
+    det = a(1,1)*(a(2,2)*a(3,3)-a(3,2)*a(2,3)) - a(1,2)*(a(2,1)*a(3,3)-a(2,3)*a(3,1)) &
+        + a(1,3)*(a(2,1)*a(3,2)-a(2,2)*a(3,1))
+    inv(1,1) = (a(2,2)*a(3,3)-a(3,2)*a(2,3))/det
+    inv(1,2) = (a(1,3)*a(3,2)-a(1,2)*a(3,3))/det
+    inv(1,3) = (a(1,2)*a(2,3)-a(1,3)*a(2,2))/det
+    inv(2,1) = (a(2,3)*a(3,1)-a(2,1)*a(3,3))/det
+    inv(2,2) = (a(1,1)*a(3,3)-a(1,3)*a(3,1))/det
+    inv(2,3) = (a(2,1)*a(1,3)-a(1,1)*a(2,3))/det
+    inv(3,1) = (a(2,1)*a(3,2)-a(3,1)*a(2,2))/det
+    inv(3,2) = (a(3,1)*a(1,2)-a(1,1)*a(3,2))/det
+    inv(3,3) = (a(1,1)*a(2,2)-a(2,1)*a(1,2))/det
+    f = matmul(inv, c)
+  end subroutine
+
+  ! one coupling block: rows in eV (after the ryd conversion)
+  subroutine read_rows(inp, name, key, r)
+    type(input_t), intent(in) :: inp
+    character(len=*), intent(in) :: name, key
+    type(rows_t), intent(out) :: r
+    logical :: found
+    integer :: first, last, k, n, p, q, ios
+    real(dp) :: econv, lconv, v3(3)
+    character(len=maxl) :: s, tok(8)
+    integer :: nt
+    call block(name, found, first, last)
+    if (.not. found) return
+    econv = 1.0_dp
+    lconv = 1.0_dp
+    if (first <= last .and. index(lines(first), 'f1=') == 0 .and. index(lines(first), 'c1=') == 0) then
+      if (index(lines(first), 'ryd') > 0) econv = hart * 0.5_dp
+      if (index(lines(first), 'bohr') > 0) lconv = bohr
+      first = first + 1
+    end if
+    n = 0
+    do k = first, last
+      if (len_trim(lines(k)) > 0) n = n + 1
+    end do
+    r%n = n
+    allocate(r%f1(3,n), r%f2(3,n), r%val(n))
+    n = 0
+    do k = first, last
+      if (len_trim(lines(k)) == 0) cycle
+      n = n + 1
+      ! remove blanks, split on ':'
+      s = ' '
+      q = 0
+      do p = 1, len_trim(lines(k))
+        if (lines(k)(p:p) /= ' ') then
+          q = q + 1
+          s(q:q) = lines(k)(p:p)
+        end if
+      end do
+      nt = 0
+      p = 1
+      do while (p <= len_trim(s) .and. nt < 8)
+        q = index(s(p:), ':')
+        nt = nt + 1
+        if (q == 0) then
+          tok(nt) = s(p:)
+          exit
+        end if
+        tok(nt) = s(p:p+q-2)
+        p = p + q
+      end do
+      if (nt < 3) call input_error('malformed coupling row: '//trim(lines(k)))
+      call site(tok(1), '1', r%f1(:,n))
+      call site(tok(2), '2', r%f2(:,n))
+      if (tok(3)(1:len(key)+1) /= key//'=') call input_error('malformed parameter: '//trim(lines(k)))
+      read(tok(3)(len(key)+2:), *, iostat=ios) r%val(n)
+      if (ios /= 0) call input_error('bad value in '//trim(lines(k)))
+      r%val(n) = r%val(n) * econv
+    end do
+  contains
+    subroutine site(t, tag, f)
+      character(len=*), intent(in) :: t, tag
+      real(dp), intent(out) :: f(3)
+      integer :: ios2
+      if (t(1:3) == 'f'//tag//'=') then
+        read(t(4:), *, iostat=ios2) f
+      else if (t(1:3) == 'c'//tag//'=') then
+        read(t(4:), *, iostat=ios2) v3
+        call cart_to_frac(inp%lat, v3 * lconv, f)
+      else
+        call input_error('atom site not recognised: '//trim(t))
+      end if
+      if (ios2 /= 0) call input_error('bad position: '//trim(t))
+    end subroutine
+  end subroutine
+
+  ! single-ion / field axes: "label ax ay az value" lines, sorted to the species order
+  subroutine read_axes(inp, name, par, ax, found)
+    type(input_t), intent(in) :: inp
+    character(len=*), intent(in) :: name
+    real(dp), allocatable, intent(out) :: par(:), ax(:,:)
+    logical, intent(out) :: found
+    logical :: fc, ff
+    integer :: first, last, fc1, fc2, k, a, used(inp%nb), ios
+    character(len=8) :: lab
+    real(dp) :: v(3), p
+    call block(name//'_cart', fc, fc1, fc2)
+    call block(name//'_frac', ff, first, last)
+    found = fc .or. ff
+    if (.not. found) return
+    if (fc) then
+      first = fc1
+      last = fc2
+    end if
+    allocate(par(inp%nb), ax(3, inp%nb))
+    par = 0
+    ax = 0
+    used = 0
+    do k = first, last
+      if (len_trim(lines(k)) == 0) cycle
+      if (lines(k)(1:3) == 'ang' .or. lines(k)(1:4) == 'bohr') cycle
+      read(lines(k), *, iostat=ios) lab, v, p
+      if (ios /= 0) call input_error('bad line in '//name//' block: '//trim(lines(k)))
+      if (ff) v = matmul(inp%lat, v)
+      v = v / sqrt(sum(v * v))
+      do a = 1, inp%nb
+        if (trim(inp%label(a)) == trim(lab) .and. used(a) == 0) then
+          used(a) = 1
+          ax(:, a) = v
+          par(a) = p
+          exit
+        end if
+        if (a == inp%nb) call input_error('too many lines for '//trim(lab)//' in '//name)
+      end do
+    end do
+  end subroutine
+
+  subroutine read_input(fname, inp)
+    character(len=*), intent(in) :: fname
+    type(input_t), intent(out) :: inp
+    logical :: found, frac
+    integer :: first, last, k, n, a, s, ios, nsp
+    character(len=maxl) :: v
+    character(len=8), allocatable :: lab(:), species(:)
+    real(dp), allocatable :: pos(:,:), mom(:)
+    real(dp) :: scale, step, x(3)
+    real(dp), allocatable :: dv(:,:)
+
+    call load(fname)
+
+    ! lattice (rows are vectors, parameters.F90:414-421)
+    call block('unit_cell_cart', found, first, last)
+    if (.not. found) call input_error('Did not find the cell information in the input file')
+    scale = 1.0_dp
+    if (lines(first)(1:4) == 'bohr') then
+      scale = bohr
+      first = first + 1
+    else if (lines(first)(1:3) == 'ang') then
+      first = first + 1
+    end if
+    if (last - first + 1 /= 3) call input_error('Not correct the cell information in the input file')
+    do k = 1, 3
+      read(lines(first + k - 1), *, iostat=ios) inp%lat(:, k)
+      if (ios /= 0) call input_error('bad unit_cell_cart line')
+    end do
+    inp%lat = inp%lat * scale
+
+    ! atoms
+    call block('atoms_frac', frac, first, last)
+    if (.not. frac) then
+      call block('atoms_cart', found, first, last)
+      if (.not. found) call input_error('No atoms_frac or atoms_cart block found')
+    end if
+    scale = 1.0_dp
+    if (lines(first)(1:4) == 'bohr') then
+      scale = bohr
+      first = first + 1
+    else if (lines(first)(1:3) == 'ang') then
+      first = first + 1
+    end if
+    n = last - first + 1
+    allocate(lab(n), pos(3, n), mom(n), species(n))
+    do k = 1, n
+      read(lines(first + k - 1), *, iostat=ios) lab(k), pos(:, k), mom(k)
+      if (ios /= 0) call input_error('bad atoms line: '//trim(lines(first + k - 1)))
+      if (.not. frac) then
+        call cart_to_frac(inp%lat, pos(:, k) * scale, x)
+        pos(:, k) = x
+      end if
+      do a = 1, 3
+        if (abs(pos(a, k) - nint(pos(a, k))) < 1e-5_dp) pos(a, k) = nint(pos(a, k))
+      end do
+      pos(:, k) = pos(:, k) - floor(pos(:, k))
+    end do
+    nsp = 0
+    do k = 1, n
+      if (all(species(1:nsp) /= lab(k))) then
+        nsp = nsp + 1
+        species(nsp) = lab(k)
+      end if
+    end do
+    inp%nb = n
+    allocate(inp%label(n), inp%basis(3, n), inp%moment(n))
+    a = 0
+    do s = 1, nsp
+      do k = 1, n
+        if (lab(k) == species(s)) then
+          a = a + 1
+          inp%label(a) = lab(k)
+          inp%basis(:, a) = pos(:, k)
+          inp%moment(a) = mom(k)
+        end if
+      end do
+    end do
+
+    call keyword('supercell_size', found, v)
+    if (found) read(v, *) inp%L
+    if (any(inp%L < 1)) call input_error('supercell_size must be greater than zero')
+
+    call keyword('tems_mode', found, v)
+    if (found) inp%tems_mode = v
+    call keyword('tem_start', found, v)
+    if (found) read(v, *) inp%tem_start
+    call keyword('tem_end', found, v)
+    if (found) read(v, *) inp%tem_end
+    call keyword('tems_num', found, v)
+    if (found) read(v, *) inp%tems_num
+    if (inp%tems_num < 1) call input_error('tems_num must be greater than zero')
+    allocate(inp%tems(inp%tems_num))
+    ! temperature ladders, parameters.F90:293-326
+    if (inp%tems_mode(1:3) == 'man') then
+      call keyword('tems', found, v)
+      if (.not. found) call input_error('tems_mode set to manual but no tems found')
+      read(v, *, iostat=ios) inp%tems
+      if (ios /= 0) call input_error('cannot read tems')
+    else
+      inp%tems(1) = inp%tem_start
+      if (inp%tems_num > 1) then
+        select case (inp%tems_mode(1:3))
+        case ('lin')
+          step = (inp%tem_end - inp%tem_start) / real(inp%tems_num - 1, dp)
+          do k = 1, inp%tems_num
+            inp%tems(k) = inp%tem_start + (k - 1) * step
+          end do
+        case ('inv')
+          step = (1.0_dp / inp%tem_end - 1.0_dp / inp%tem_start) / real(inp%tems_num - 1, dp)
+          do k = 1, inp%tems_num
+            inp%tems(k) = 1.0_dp / (1.0_dp / inp%tem_start + (k - 1) * step)
+          end do
+        case ('log')
+          do k = 1, inp%tems_num
+            inp%tems(k) = inp%tem_start * exp(log(inp%tem_end / inp%tem_start) * &
+                          (1.0_dp * (k - 1) / (inp%tems_num - 1)))
+          end do
+        case default
+          call input_error('tems_mode not recognised')
+        end select
+      end if
+    end if
+    if (any(inp%tems <= 0)) call input_error('tems must be greater than zero')
+
+    call keyword('steps_warmup', found, v)
+    if (found) read(v, *) inp%steps_warmup
+    if (inp%steps_warmup < 0) inp%steps_warmup = 0
+    call keyword('steps_mc', found, v)
+    if (found) read(v, *) inp%steps_mc
+    if (inp%steps_mc < 1) call input_error('steps_mc must be greater than zero')
+    call keyword('steps_measure', found, v)
+    if (found) read(v, *) inp%steps_measure
+    if (inp%steps_measure < 1) call input_error('steps_measure must be greater than zero')
+    call keyword('initial_sconfig', found, v)
+    if (found) inp%initial = v
+    if (inp%initial(1:5) /= 'ferro' .and. inp%initial(1:4) /= 'rand' .and. inp%initial(1:4) /= 'file') &
+      call input_error('Initial_sconfig not recognised')
+    call keyword('mcarlo_mode', found, v)
+    if (found) inp%mcarlo_mode = v
+    if (inp%mcarlo_mode(1:5) /= 'const' .and. inp%mcarlo_mode(1:4) /= 'rand') &
+      call input_error('mcarlo_mode not recognised')
+    call keyword('tilt_angles_max', found, v)
+    if (found) read(v, *) inp%tilt
+    call keyword('seeds', found, v)
+    if (found) then
+      read(v, *) inp%seed
+      inp%have_seed = .true.
+    end if
+    call keyword('pt', found, v)
+    if (found) inp%pt = is_true(v)
+    if (inp%pt .and. inp%tems_num <= 1) call input_error('tems_num must be greater than one in parallel tempering')
+    call keyword('pt_steps_swap', found, v)
+    if (found) read(v, *) inp%pt_steps_swap
+    if (inp%pt .and. inp%pt_steps_swap < 1) call input_error('pt_steps_swap must be greater than zero')
+    call keyword('pt_print_swap', found, v)
+    if (found) read(v, *) inp%pt_print_swap
+    call keyword('boundary', found, v)
+    if (found) inp%open_boundary = v(1:4) == 'open'
+    if (inp%pt) call sort(inp%tems)
+
+    ! Hamiltonian
+    call read_rows(inp, 'jij_parameters', 'jij', inp%J)
+    call keyword('ham_bij', found, v)
+    if (found) inp%have_B = is_true(v)
+    call keyword('ham_dij', found, v)
+    if (found) inp%have_D = is_true(v)
+    if (inp%have_B) call read_rows(inp, 'bij_parameters', 'bij', inp%B)
+    if (inp%have_D) then
+      call read_rows(inp, 'dij_parameters', 'dij', inp%D)
+      call block('dij_vectors_cart', found, first, last)
+      frac = .false.
+      if (.not. found) then
+        call block('dij_vectors_frac', found, first, last)
+        frac = found
+      end if
+      if (.not. found) call input_error('no dij_vectors_frac or dij_vectors_cart block found')
+      if (lines(first)(1:3) == 'ang' .or. lines(first)(1:4) == 'bohr') first = first + 1
+      if (last - first + 1 /= inp%D%n) call input_error('Wrong number of lines in dij_vectors block')
+      allocate(dv(3, inp%D%n))
+      do k = 1, inp%D%n
+        read(lines(first + k - 1), *) dv(:, k)
+        if (frac) dv(:, k) = matmul(inp%lat, dv(:, k))
+        dv(:, k) = dv(:, k) / sqrt(sum(dv(:, k)**2))
+      end do
+      call move_alloc(dv, inp%D%dvec)
+    end if
+    call keyword('ham_singleion', found, v)
+    if (found) inp%have_si = is_true(v)
+    if (inp%have_si) then
+      call read_axes(inp, 'singleion_axes', inp%si_par, inp%si_axis, found)
+      if (.not. found) call input_error('ham_singleion is true but no singleion_axes block found')
+    end if
+    call keyword('ham_field', found, v)
+    if (found) inp%have_h = is_true(v)
+    if (inp%have_h) then
+      call read_axes(inp, 'field_axes', inp%h_par, inp%h_axis, found)
+      if (.not. found) call input_error('ham_field is true but no field_axes block found')
+    end if
+  contains
+    subroutine sort(t)
+      real(dp), intent(inout) :: t(:)
+      integer :: i, j
+      real(dp) :: tmp
+      do i = 2, size(t)
+        tmp = t(i)
+        j = i - 1
+        do while (j >= 1)
+          if (t(j) <= tmp) exit
+          t(j + 1) = t(j)
+          j = j - 1
+        end do
+        t(j + 1) = tmp
+      end do
+    end subroutine
+  end subroutine read_input
+
+end module mc_input

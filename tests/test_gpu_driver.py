@@ -1,0 +1,86 @@
+"""The Fortran driver (espins_amd/driver/mc_gpu.x) end to end on the reference's example inputs:
+same .mcin in, same _mc.dat / .mcout / _sconfig.dat layout out."""
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests.conftest import GOLDEN, ROOT
+
+pytestmark = pytest.mark.gpu
+EXE = os.path.join(ROOT, "espins_amd", "driver", "mc_gpu.x")
+
+
+def _prepare(tmp_path, src, seed, edits):
+    text = open(os.path.join(GOLDEN, "examples", src)).read()
+    for key, val in edits.items():
+        pat = re.compile(rf"^\s*!?!?\s*{key}\s*[=:].*$", re.I | re.M)
+        if pat.search(text):
+            text = pat.sub(f" {key} = {val}", text, count=1)
+        else:
+            text = f" {key} = {val}\n" + text
+    p = tmp_path / f"{seed}.mcin"
+    p.write_text(text)
+    return p
+
+
+def _run(tmp_path, seed):
+    assert os.path.exists(EXE), "driver not built"
+    r = subprocess.run([EXE, seed], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    return (tmp_path / f"{seed}.mcout").read_text()
+
+
+def test_example01_plain_mc(tmp_path):
+    _prepare(tmp_path, "ex01_cubic.mcin", "cubic",
+             {"steps_warmup": 4000, "steps_mc": 20000, "seeds": 12345})
+    out = _run(tmp_path, "cubic")
+    # the published initial (ferro) energy, printed in the same E15.7 format
+    # (examples/example01/reference/cubic.mcout:110)
+    assert out.count("-0.6684205E+04  <-- ENERGY") == 20
+    adds = [float(x) for x in re.findall(r"adding differences\s*=\s*(\S+)", out)]
+    direct = [float(x) for x in re.findall(r"direct calculations\s*=\s*(\S+)", out)]
+    assert len(adds) == 20 and np.allclose(adds, direct, rtol=1e-6)
+    mine = np.loadtxt(tmp_path / "cubic_mc.dat", comments="#")
+    ref = np.loadtxt(os.path.join(GOLDEN, "examples", "ex01_cubic_mc.dat"), comments="#")
+    assert mine.shape == ref.shape == (20, 7)
+    assert np.allclose(mine[:, 0], ref[:, 0], atol=1e-4)
+    # energy per site within 2 % of the published run at every temperature
+    assert np.allclose(mine[:, 2], ref[:, 2], rtol=0.02)
+    # _sconfig.dat: header, N, T, then unit spins (moment 1)
+    lines = (tmp_path / "cubic_sconfig.dat").read_text().splitlines()
+    assert lines[1].split() == ["4", "4", "4"] and lines[2].split() == ["64", "20"]
+    s = np.array([[float(v) for v in l.split()] for l in lines[4:68]])
+    assert np.allclose(np.linalg.norm(s, axis=1), 1.0, atol=2e-6)
+
+
+def test_sconfig_restart(tmp_path):
+    _prepare(tmp_path, "ex01_cubic.mcin", "cubic",
+             {"steps_warmup": 200, "steps_mc": 400, "seeds": 7})
+    _run(tmp_path, "cubic")
+    final = [float(x) for x in re.findall(r"direct calculations\s*=\s*(\S+)",
+                                          (tmp_path / "cubic.mcout").read_text())]
+    _prepare(tmp_path, "ex01_cubic.mcin", "cubic",
+             {"steps_warmup": 0, "steps_mc": 10, "seeds": 8, "initial_sconfig": "file"})
+    out = _run(tmp_path, "cubic")
+    init = [float(x) for x in re.findall(r"Initial Energy\s*=\s*(\S+)", out)]
+    # restart energies agree with the energies the first run ended with (3F12.6 spins)
+    assert np.allclose(init, final, rtol=2e-4)
+
+
+def test_example02_parallel_tempering(tmp_path):
+    _prepare(tmp_path, "ex02_cubic.mcin", "cubic",
+             {"steps_warmup": 400, "steps_mc": 1000, "supercell_size": "6 6 6", "seeds": 3,
+              "energy_write": ".false."})
+    out = _run(tmp_path, "cubic")
+    mine = np.loadtxt(tmp_path / "cubic_mc.dat", comments="#")
+    assert mine.shape == (32, 7)
+    assert np.all(np.diff(mine[:, 0]) > 0)
+    m = re.findall(r"Final configuration =\s*([\d\s]+)", out)
+    perm = [int(x) for x in m[-1].split()]
+    assert sorted(perm) == list(range(1, 33))
+    assert "swaps" in out
