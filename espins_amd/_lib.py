@@ -53,6 +53,7 @@ _SIGS = {
                            _i, _dp, _dp, _i, _dp, _dp]),
     "esp_set_supercell": (_i, [_p, _ip]),
     "esp_set_path": (_i, [_p, _i]),
+    "esp_set_stream_base": (_i, [_p, C.c_uint32]),
     "esp_set_tps": (_i, [_p, _i]),
     "esp_plan_model": (_i, [_i, _i, _dp, _i, _ip, _ip, _dp, _i, _ip, _ip, _dp, _i, _ip, _ip, _dp,
                             _ip, _ip, _ip, _ip]),
@@ -74,6 +75,27 @@ _SIGS = {
     "esp_sync": (_i, [_p]),
     "esp_stream": (_p, [_p]),
 }
+
+_SIGS.update({
+    "esp_batch_create": (_i, [C.POINTER(_p), _i]),
+    "esp_batch_destroy": (_i, [_p]),
+    "esp_batch_last_error": (C.c_char_p, [_p]),
+    "esp_batch_add_chain": (_i, [_p, _i, _i, _dp, _i, _ip, _ip, _dp, _i, _ip, _ip, _dp,
+                                 _i, _ip, _ip, _dp, _i, _dp, _dp, _i, _dp, _dp,
+                                 _ip, _i, _dp, _dp, _ip]),
+    "esp_batch_set_chain_config": (_i, [_p, _i, _i, _i]),
+    "esp_batch_commit": (_i, [_p]),
+    "esp_batch_autotune": (_i, [_p, _i, C.c_char_p]),
+    "esp_batch_run": (_i, [_p, _i, _i, _i, _i, _i, _i, _i, _d, _u64, _u64, _u64, _i]),
+    "esp_batch_num_chains": (_i, [_p]),
+    "esp_batch_info": (_i, [_p, _i, _ip]),
+    "esp_batch_get_stats": (_i, [_p, _i, _dp, _i64p, _dp, _i64p]),
+    "esp_batch_get_map": (_i, [_p, _i, _ip, _i64p]),
+    "esp_batch_get_spins": (_i, [_p, _i, _i, _dp]),
+    "esp_batch_energy": (_i, [_p, _i, _dp]),
+    "esp_batch_reset_stats": (_i, [_p]),
+    "esp_batch_sync": (_i, [_p]),
+})
 
 OPTIONAL = set()
 

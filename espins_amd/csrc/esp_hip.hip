@@ -35,6 +35,7 @@ struct esp_ctx {
   size_t lds_max = 65536;
   int Lhint[3] = {0, 0, 0};       // supercell size for the periodic colouring
   int path = 0;                   // 0 auto, 1 global-memory phases, 2 LDS-resident
+  uint32_t stream_base = 0;       // Philox stream of replica r: stream_base + r
   bool lds_fits = false;
   // model
   esp::HostModel hm;
@@ -142,6 +143,7 @@ KArgs base_args(esp_ctx *c) {
   a.beta_slot = c->d_beta; a.rep_slot = c->d_rep_slot; a.rej = c->d_rej; a.clock = c->d_clock;
   a.N = m.N; a.Ns = c->Ns; a.nb = m.nb; a.zmax = m.zmax; a.R = c->R; a.ncol = m.ncol;
   a.has_si = m.has_si; a.has_h = m.has_h;
+  a.stream_base = c->stream_base;
   return a;
 }
 
@@ -198,7 +200,6 @@ int enqueue_phase_sweeps(esp_ctx *c, int mode, double tilt, uint64_t seed, int n
   KArgs a = base_args(c);
   a.tilt = tilt;
   a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32);
-  a.stream_base = 0;
   PhaseFn fn = phase_fn(mode, m);
   const size_t shm = m.compressed ? c->ctab_bytes : 0;
   for (int s = 0; s < nsweeps; ++s)
@@ -245,7 +246,7 @@ int enqueue_finalize(esp_ctx *c, bool use_partials, bool measure, bool exchange,
   f.slot_rep = c->d_slot_rep; f.rep_slot = c->d_rep_slot; f.beta_slot = c->d_beta;
   f.nswap = c->d_nswap; f.clock = c->d_clock;
   f.measure = measure; f.exchange = exchange; f.advance_sweeps = advance;
-  f.key0 = (uint32_t)seed; f.key1 = (uint32_t)(seed >> 32); f.stream_base = 0;
+  f.key0 = (uint32_t)seed; f.key1 = (uint32_t)(seed >> 32); f.stream_base = c->stream_base;
   const int nt = std::min(1024, std::max(64, ((c->R + 63) / 64) * 64));
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(nt), 0, c->stream, f);
   HIPCHK(c, hipGetLastError());
@@ -265,8 +266,9 @@ int enqueue_energy(esp_ctx *c, double *d_out) {
   KArgs a = base_args(c);
   EnergyFn fn = energy_fn(m.compressed, m.has_B, m.has_D);
   const int nblk = c->red_blocks;
-  hipLaunchKernelGGL(fn, dim3(nblk, c->R), dim3(kBlock), m.compressed ? c->ctab_bytes : 0,
-                     c->stream, a, nblk, c->d_partial);
+  hipLaunchKernelGGL(fn, dim3(nblk, c->R), dim3(kBlock),
+                     (m.compressed ? c->ctab_bytes : 0) + (kBlock / 64) * sizeof(double), c->stream, a,
+                     nblk, c->d_partial);
   hipLaunchKernelGGL(k_energy_fin, dim3((c->R + 255) / 256), dim3(256), 0, c->stream,
                      (const double *)c->d_partial, nblk, c->R, d_out);
   HIPCHK(c, hipGetLastError());
@@ -445,7 +447,7 @@ int esp_set_model(esp_ctx *c, int N, int nb, const double *mag, int zJ, const in
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->ctab_bytes = m.compressed ? ctab_bytes : 0;
-  c->lds_bytes = (size_t)(N + 1) * sizeof(double4) + c->ctab_bytes;
+  c->lds_bytes = (size_t)(N + 1) * sizeof(double4) + c->ctab_bytes + kLdsRed * sizeof(double);
   c->Ns = ((N + 1 + 63) / 64) * 64;
   c->tps = c->tps_user ? c->tps_user : (m.zmax <= 8 ? 1 : (m.zmax <= 20 ? 2 : 4));
   c->lds_fits = c->lds_bytes + 4096 <= c->lds_max;
@@ -504,6 +506,13 @@ int esp_set_tps(esp_ctx *c, int tps) {
     c->tps = tps ? tps : (z <= 8 ? 1 : (z <= 20 ? 2 : 4));
     drop_graph(c);
   }
+  return ESP_OK;
+}
+
+int esp_set_stream_base(esp_ctx *c, uint32_t base) {
+  if (!c) return ESP_EINVAL;
+  c->stream_base = base;
+  drop_graph(c);
   return ESP_OK;
 }
 

@@ -13,11 +13,15 @@
 #include "esp_device.h"
 
 namespace espk {
+namespace {  // internal linkage: each translation unit that launches these owns its copies
 using namespace espd;
 
 constexpr int kBlock = 256;      // phase / reduce kernels (4 waves)
 constexpr int kLdsBlock = 512;   // LDS-resident kernel (8 waves per replica)
 constexpr int kChunk = 8;        // partner gathers issued together per lane
+// Dynamic LDS of the LDS-resident kernel: [N+1 spins as double4][coefficient table]
+// [kLdsRed doubles of reduction scratch]; every byte of LDS it uses is in that one region.
+constexpr int kLdsRed = 4 * (512 / 64);
 
 struct Clock {                   // device-side schedule counters (graph-replayable)
   unsigned long long sweep;      // Philox sweep counter of the next sweep
@@ -223,13 +227,10 @@ __device__ __forceinline__ void decode_block(const KArgs &a, int &r, int &b) {
   }
 }
 
-// One colour phase over every replica (global-memory path).  a.tps lanes per site.
+// Block b of colour phase (a.c_begin, a.c_count) of replica r (global-memory path);
+// a.tps lanes per site.  Shared by the single-context and the batched kernels.
 template <int MODE, bool CMP, bool HB, bool HD, bool SELF>
-__global__ __launch_bounds__(kBlock) void k_phase(KArgs a) {
-  extern __shared__ double s_ctab[];
-  int r, b;
-  decode_block(a, r, b);
-  if (r >= a.R) return;  // uniform per block
+__device__ __forceinline__ void phase_body(const KArgs &a, int r, int b, double *s_ctab) {
   if (CMP) {
     const int n = a.nb * a.zmax * 5;
     for (int k = threadIdx.x; k < n; k += kBlock) s_ctab[k] = a.ctab[k];
@@ -268,6 +269,16 @@ __global__ __launch_bounds__(kBlock) void k_phase(KArgs a) {
   if ((threadIdx.x & 63) == 0 && m) atomicAdd(&a.rej[slot], (unsigned long long)__popcll(m));
 }
 
+// One colour phase over every replica of one context.
+template <int MODE, bool CMP, bool HB, bool HD, bool SELF>
+__global__ __launch_bounds__(kBlock) void k_phase(KArgs a) {
+  extern __shared__ double s_ctab[];
+  int r, b;
+  decode_block(a, r, b);
+  if (r >= a.R) return;  // uniform per block
+  phase_body<MODE, CMP, HB, HD, SELF>(a, r, b, s_ctab);
+}
+
 struct LdsMeas {
   double *acc;                   // [R][6] by slot
   double *e_run;                 // [R] by replica
@@ -279,11 +290,10 @@ struct LdsMeas {
 // All colours of nsweeps sweeps of one replica per block, spins resident in LDS,
 // measurement (mcarlo.F90:536-543) fused every measure_every sweeps.
 template <int MODE, bool CMP, bool HB, bool HD, bool SELF>
-__global__ __launch_bounds__(kLdsBlock) void k_lds(KArgs a, int nsweeps, LdsMeas m) {
-  extern __shared__ double4 s_sp[];
-  __shared__ double red[4 * (kLdsBlock / 64)];
-  const int r = blockIdx.x;
+__device__ __forceinline__ void lds_body(const KArgs &a, int r, int nsweeps, const LdsMeas &m,
+                                         double4 *s_sp) {
   double *ctab = reinterpret_cast<double *>(s_sp + a.N + 1);
+  double *red = ctab + (CMP ? a.nb * a.zmax * 5 : 0);
   const double *gx = a.sp + (size_t)r * 3 * a.Ns;
   double *gw = a.w + (size_t)r * a.Ns;
   for (int d = threadIdx.x; d <= a.N; d += kLdsBlock)   // index N: the zero spin
@@ -300,6 +310,15 @@ __global__ __launch_bounds__(kLdsBlock) void k_lds(KArgs a, int nsweeps, LdsMeas
   const int g = threadIdx.x & (tps - 1);
   const int per = kLdsBlock / tps;
   unsigned int nrej = 0;
+  // The estimator sums of this replica's slot live in thread 0's registers during the
+  // launch (same additions, same order as adding to memory each time) and are stored once
+  // at the end: the sweep loop itself writes nothing but LDS.
+  double A[6] = {0, 0, 0, 0, 0, 0};
+  double e_last = 0.0;
+  unsigned nmeas_local = 0;
+  if (threadIdx.x == 0 && m.measure_every > 0)
+    for (int k = 0; k < 6; ++k) A[k] = m.acc[6 * slot + k];
+  const double e_base = m.e_base[r];
   for (int sw = 0; sw < nsweeps; ++sw) {
     const unsigned long long sweep = sweep0 + (unsigned long long)sw;
     for (int c = 0; c < a.ncol; ++c) {
@@ -339,20 +358,26 @@ __global__ __launch_bounds__(kLdsBlock) void k_lds(KArgs a, int nsweeps, LdsMeas
       }
       block_sum<4>(v, red);
       if (threadIdx.x == 0) {
-        const double e = m.e_base[r] + v[0];
+        const double e = e_base + v[0];
         if (meas) {
           const double n = (double)a.N;
           const double mx = v[1] / n, my = v[2] / n, mz = v[3] / n;
           const double m2 = mx * mx + my * my + mz * mz;
-          double *acc = m.acc + 6 * slot;
           const double e2 = e * e;
-          acc[0] += e; acc[1] += e2; acc[2] += e2 * e2;
-          acc[3] += sqrt(m2); acc[4] += m2; acc[5] += m2 * m2;
-          if (r == 0) m.nmeas[0] += 1;
+          A[0] += e; A[1] += e2; A[2] += e2 * e2;
+          A[3] += sqrt(m2); A[4] += m2; A[5] += m2 * m2;
+          ++nmeas_local;
         }
-        if (last) m.e_run[r] = e;
+        e_last = e;
       }
     }
+  }
+  if (threadIdx.x == 0) {
+    if (nmeas_local) {
+      for (int k = 0; k < 6; ++k) m.acc[6 * slot + k] = A[k];
+      if (r == 0) m.nmeas[0] += nmeas_local;
+    }
+    if (nsweeps > 0) m.e_run[r] = e_last;
   }
   double *ox = a.sp + (size_t)r * 3 * a.Ns;
   for (int d = threadIdx.x; d < a.N; d += kLdsBlock) {
@@ -362,6 +387,12 @@ __global__ __launch_bounds__(kLdsBlock) void k_lds(KArgs a, int nsweeps, LdsMeas
   unsigned int v = nrej;
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(&a.rej[slot], (unsigned long long)v);
+}
+
+template <int MODE, bool CMP, bool HB, bool HD, bool SELF>
+__global__ __launch_bounds__(kLdsBlock) void k_lds(KArgs a, int nsweeps, LdsMeas m) {
+  extern __shared__ double4 s_sp[];
+  lds_body<MODE, CMP, HB, HD, SELF>(a, blockIdx.x, nsweeps, m, s_sp);
 }
 
 // Partial sums of (w, x, y, z) per (replica, block) for the running energy and m.
@@ -398,8 +429,9 @@ struct FinArgs {
 };
 
 // Running energies / measurement, then the replica exchange (pt_swap, pt.F90:1235-1288)
-// in label form, then the schedule clock.  One block.
-__global__ __launch_bounds__(1024) void k_finalize(FinArgs f) {
+// in label form.  One block per context; start parity and step come from the clock values
+// the caller read.
+__device__ __forceinline__ void finalize_body(const FinArgs &f, int start, unsigned long long step) {
   if (f.partial)
     for (int r = threadIdx.x; r < f.R; r += blockDim.x) {
       double v[4] = {0, 0, 0, 0};
@@ -420,8 +452,6 @@ __global__ __launch_bounds__(1024) void k_finalize(FinArgs f) {
     }
   __syncthreads();
   if (f.exchange) {
-    const int start = f.clock->parity;  // 1 or 2 (pt.F90:679,898)
-    const unsigned long long step = f.clock->step;
     for (int t = start - 1 + 2 * threadIdx.x; t + 1 < f.R; t += 2 * blockDim.x) {
       const int ra = f.slot_rep[t], rb = f.slot_rep[t + 1];
       const double fac = (f.beta_slot[t] - f.beta_slot[t + 1]) * (f.e_run[ra] - f.e_run[rb]);
@@ -443,21 +473,29 @@ __global__ __launch_bounds__(1024) void k_finalize(FinArgs f) {
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    if (f.measure) f.nmeas[0] += 1;
-    if (f.exchange) {
-      f.clock->step += 1;
-      f.clock->parity = 3 - f.clock->parity;
-    }
-    f.clock->sweep += (unsigned long long)f.advance_sweeps;
+  if (threadIdx.x == 0 && f.measure) f.nmeas[0] += 1;
+}
+
+__device__ __forceinline__ void advance_clock(Clock *c, bool exchange, int advance_sweeps) {
+  if (exchange) {
+    c->step += 1;
+    c->parity = 3 - c->parity;
   }
+  c->sweep += (unsigned long long)advance_sweeps;
+}
+
+__global__ __launch_bounds__(1024) void k_finalize(FinArgs f) {
+  const int start = f.clock->parity;  // 1 or 2 (pt.F90:679,898)
+  const unsigned long long step = f.clock->step;
+  finalize_body(f, start, step);
+  if (threadIdx.x == 0) advance_clock(f.clock, f.exchange, f.advance_sweeps);
 }
 
 // Direct energy (energy_get, get_quant.F90:186-261): per-site terms, block partials.
 template <bool CMP, bool HB, bool HD>
 __global__ __launch_bounds__(kBlock) void k_energy(KArgs a, int nblk, double *partial) {
-  extern __shared__ double s_ctab[];
-  __shared__ double red[kBlock / 64];
+  extern __shared__ double s_ctab[];   // [coefficient table][kBlock/64 scratch]
+  double *red = s_ctab + (CMP ? a.nb * a.zmax * 5 : 0);
   const int r = blockIdx.y, bk = blockIdx.x;
   if (CMP) {
     for (int k = threadIdx.x; k < a.nb * a.zmax * 5; k += kBlock) s_ctab[k] = a.ctab[k];
@@ -547,4 +585,5 @@ __global__ void k_fill(double *p, size_t n, double v) {
   if (t < n) p[t] = v;
 }
 
+}  // namespace
 }  // namespace espk

@@ -328,7 +328,8 @@ std::string prepare_model(HostModel &hm, int N, int nb, const double *mag, int z
                           const int *b_cnt, const int *b_nbr, const double *b_val, int zD,
                           const int *d_cnt, const int *d_nbr, const double *d_vec, int have_si,
                           const double *si_par, const double *si_axis, int have_field,
-                          const double *h_par, const double *h_axis, const int *Lhint) {
+                          const double *h_par, const double *h_axis, const int *Lhint,
+                          const std::vector<int> *colour_hint) {
   if (N <= 0 || nb <= 0 || N % nb != 0) return "N must be a positive multiple of nb";
   if (!mag) return "mag is NULL";
   if (zJ < 0 || zB < 0 || zD < 0) return "negative table width";
@@ -396,7 +397,14 @@ std::string prepare_model(HostModel &hm, int N, int nb, const double *mag, int z
     if (c > 0 && c * c * c == nc) Lc[0] = Lc[1] = Lc[2] = c;
   }
   hm.colour.clear();
-  if (Lc[0] > 0) hm.colour = periodic_colouring(N, nb, Lc, adj);
+  if (colour_hint && (int)colour_hint->size() == N) {  // a colouring of the same structure
+    bool ok = true;
+    for (int i = 0; i < N && ok; ++i)
+      for (int j : adj[i])
+        if ((*colour_hint)[j] == (*colour_hint)[i]) { ok = false; break; }
+    if (ok) hm.colour = *colour_hint;
+  }
+  if (hm.colour.empty() && Lc[0] > 0) hm.colour = periodic_colouring(N, nb, Lc, adj);
   if (hm.colour.empty()) hm.colour = greedy_colouring(N, adj);
   int ncol = 0;
   for (int c : hm.colour) ncol = std::max(ncol, c + 1);
@@ -475,6 +483,22 @@ std::string prepare_model(HostModel &hm, int N, int nb, const double *mag, int z
   }
   if (nb > 255) return "more than 255 basis atoms";
   return "";
+}
+
+uint64_t structure_hash(int N, int nb, int zJ, const int *j_cnt, const int *j_nbr, int zB,
+                        const int *b_cnt, const int *b_nbr, int zD, const int *d_cnt,
+                        const int *d_nbr) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](const void *p, size_t n) {
+    const unsigned char *c = static_cast<const unsigned char *>(p);
+    for (size_t k = 0; k < n; ++k) { h ^= c[k]; h *= 1099511628211ull; }
+  };
+  const int head[5] = {N, nb, zJ, zB, zD};
+  mix(head, sizeof head);
+  if (zJ > 0) { mix(j_cnt, sizeof(int) * N); mix(j_nbr, sizeof(int) * (size_t)zJ * N); }
+  if (zB > 0) { mix(b_cnt, sizeof(int) * N); mix(b_nbr, sizeof(int) * (size_t)zB * N); }
+  if (zD > 0) { mix(d_cnt, sizeof(int) * N); mix(d_nbr, sizeof(int) * (size_t)zD * N); }
+  return h;
 }
 
 }  // namespace esp

@@ -42,7 +42,13 @@ std::string prepare_model(HostModel &hm, int N, int nb, const double *mag,
                           int zD, const int *d_cnt, const int *d_nbr, const double *d_vec,
                           int have_si, const double *si_par, const double *si_axis,
                           int have_field, const double *h_par, const double *h_axis,
-                          const int *Lhint /* supercell size or NULL */);
+                          const int *Lhint /* supercell size or NULL */,
+                          const std::vector<int> *colour_hint = nullptr);
+
+// 64-bit FNV-1a over the bond structure (counts and partner lists) of a model.
+uint64_t structure_hash(int N, int nb, int zJ, const int *j_cnt, const int *j_nbr, int zB,
+                        const int *b_cnt, const int *b_nbr, int zD, const int *d_cnt,
+                        const int *d_nbr);
 
 // jij_get-style ELL expansion (see espins_hip.h, esp_build_table).
 int build_table(int nb, const double *basis_frac, const double *basis_mag, const int *L,

@@ -311,3 +311,132 @@ def plan_model(m: Model):
                                  C.byref(n), E.ptr(col, i), E.ptr(info, i)))
     return n.value, col, dict(N=int(info[0]), zmax=int(info[1]), ncol=int(info[2]),
                               compressed=bool(info[3]))
+
+
+class Batch:
+    """Many independent PT chains advanced together (esp_batch_* in include/espins_hip.h)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        rc = E.lib.esp_batch_create(C.byref(h), device)
+        if rc != E.ESP_OK:
+            raise E.EspError(rc, f"esp_batch_create(device={device}) failed (is a GPU visible?)")
+        self._h = h
+        self.chains = []          # (N, R, tems)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            E.lib.esp_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        if rc != E.ESP_OK:
+            raise E.EspError(rc, E.lib.esp_batch_last_error(self._h).decode())
+
+    def add_chain(self, m: Model, tems, spins):
+        d, i = C.c_double, C.c_int
+
+        def tab(t):
+            if t is None or t.z == 0:
+                return 0, None, None, None
+            return t.z, E.i32(t.cnt), E.i32(t.nbr), E.f64(t.val)
+
+        mag = E.f64(m.mag)
+        zJ, jc, jn, jv = tab(m.J)
+        zB, bc, bn, bv = tab(m.B)
+        zD, dc, dn, dv = tab(m.D)
+        sip = E.f64(m.si_par) if m.si_par is not None else None
+        sia = E.f64(m.si_axis) if m.si_axis is not None else None
+        hp = E.f64(m.h_par) if m.h_par is not None else None
+        ha = E.f64(m.h_axis) if m.h_axis is not None else None
+        Ls = np.array(m.L, np.int32) if getattr(m, "L", None) is not None else None
+        tems = E.f64(tems)
+        spins = E.f64(spins)
+        assert spins.shape == (len(tems), m.N, 3)
+        cid = C.c_int()
+        self._chk(E.lib.esp_batch_add_chain(
+            self._h, m.N, m.nb, E.ptr(mag, d),
+            zJ, E.ptr(jc, i), E.ptr(jn, i), E.ptr(jv, d),
+            zB, E.ptr(bc, i), E.ptr(bn, i), E.ptr(bv, d),
+            zD, E.ptr(dc, i), E.ptr(dn, i), E.ptr(dv, d),
+            int(sip is not None), E.ptr(sip, d), E.ptr(sia, d),
+            int(hp is not None), E.ptr(hp, d), E.ptr(ha, d),
+            E.ptr(Ls, i), len(tems), E.ptr(tems, d), E.ptr(spins, d), C.byref(cid)))
+        self.chains.append((m.N, len(tems), tems.copy()))
+        return cid.value
+
+    def set_chain_config(self, k, path, tps):
+        self._chk(E.lib.esp_batch_set_chain_config(self._h, int(k), int(path), int(tps)))
+
+    def commit(self):
+        self._chk(E.lib.esp_batch_commit(self._h))
+        return self
+
+    def autotune(self, nsweeps=4, cache=None):
+        self._chk(E.lib.esp_batch_autotune(self._h, int(nsweeps),
+                                           cache.encode() if cache else None))
+
+    def run(self, n_rounds, steps_per_round=1, steps_measure=1, measure=True, exchange=True,
+            start_parity=1, mode="random", tilt=0.125, seed=1, sweep0=0, step0=0, graph=True):
+        self._chk(E.lib.esp_batch_run(self._h, int(n_rounds), int(steps_per_round),
+                                      int(steps_measure), int(measure), int(exchange),
+                                      int(start_parity), _mode(mode), float(tilt),
+                                      int(seed) & (2**64 - 1), int(sweep0), int(step0), int(graph)))
+
+    def info(self, k):
+        out = np.zeros(8, np.int32)
+        self._chk(E.lib.esp_batch_info(self._h, int(k), E.ptr(out, C.c_int)))
+        return dict(N=int(out[0]), R=int(out[1]), ncol=int(out[2]), tps=int(out[3]),
+                    lds=bool(out[4]), zmax=int(out[5]), variant=int(out[6]), lds_fits=bool(out[7]))
+
+    def stats(self, k):
+        N, R, _ = self.chains[k]
+        acc = np.zeros((R, 6))
+        rej = np.zeros(R, np.int64)
+        erun = np.zeros(R)
+        nm = np.zeros(1, np.int64)
+        self._chk(E.lib.esp_batch_get_stats(self._h, int(k), E.ptr(acc, C.c_double),
+                                            E.ptr(rej, C.c_int64), E.ptr(erun, C.c_double),
+                                            E.ptr(nm, C.c_int64)))
+        return dict(acc=acc, rejected=rej, e_run=erun, nmeas=int(nm[0]))
+
+    def pt_map(self, k):
+        N, R, _ = self.chains[k]
+        sr = np.zeros(R, np.int32)
+        ns = np.zeros((R, 2), np.int64)
+        self._chk(E.lib.esp_batch_get_map(self._h, int(k), E.ptr(sr, C.c_int), E.ptr(ns, C.c_int64)))
+        return sr, ns
+
+    def spins(self, k, by_slot=False):
+        N, R, _ = self.chains[k]
+        out = np.zeros((R, N, 3))
+        self._chk(E.lib.esp_batch_get_spins(self._h, int(k), int(by_slot), E.ptr(out, C.c_double)))
+        return out
+
+    def energy_get(self, k):
+        N, R, _ = self.chains[k]
+        out = np.zeros(R)
+        self._chk(E.lib.esp_batch_energy(self._h, int(k), E.ptr(out, C.c_double)))
+        return out
+
+    def reset_stats(self):
+        self._chk(E.lib.esp_batch_reset_stats(self._h))
+
+    def sync(self):
+        self._chk(E.lib.esp_batch_sync(self._h))
+
+
+def set_stream_base(eng: Engine, base: int):
+    eng._chk(E.lib.esp_set_stream_base(eng._h, int(base) & 0xFFFFFFFF))

@@ -247,3 +247,40 @@ def config_c5(L=32, R=64):
     """Frustrated FCC: J1 = -1, J2 = +0.1, NN biquadratic 0.1, single-ion Delta = -0.05;
     geometric ladder 0.02-0.8 J."""
     return (cubic_model("fcc", L, J=(-1.0, 0.1), B=0.1, K=-0.05), ladder("log", 0.02, 0.8, R))
+
+
+# ---- the batched workload: many uneven PT chains ------------------------------------------
+_STRUCT_CACHE: dict = {}
+
+
+def cubic_model_fast(kind: str, L: int, J=(1.0,)):
+    """cubic_model for exchange-only models, reusing the bond structure of an earlier call with
+    the same (kind, L, number of shells): slots are built once with marker values k+1 for shell k
+    and relabelled with the couplings (the table builder's output does not depend on the values
+    otherwise, except that zero couplings are dropped -- callers pass non-zero J)."""
+    key = (kind, L, len(J))
+    if key not in _STRUCT_CACHE:
+        base = cubic_model(kind, L, J=tuple(float(k + 1) for k in range(len(J))))
+        _STRUCT_CACHE[key] = base
+    base = _STRUCT_CACHE[key]
+    shell = np.rint(base.J.val).astype(int) - 1
+    val = np.asarray(J, float)[shell]
+    t = Table(base.J.z, base.J.cnt, base.J.nbr, np.ascontiguousarray(val))
+    return Model(kind, base.L, base.nb, base.basis_frac, base.mag, t,
+                 rows={"J": shell_rows(kind, list(J))})
+
+
+def config_c3(n_chains=256, seed=7, Ls=(8, 12, 16, 24, 32, 48), kinds=("sc", "bcc", "fcc"),
+              r_range=(16, 32), t_range=(0.5, 2.5)):
+    """Batched uneven workload: per chain a lattice kind and L drawn uniformly, J1 = 1 ferro and
+    J2 ~ U[-0.3, 0.3], R ~ U{16..32} replicas on a geometric ladder over t_range.
+    Returns a list of (kind, L, J2, R, tems) drawn from one master seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_chains):
+        kind = kinds[int(rng.integers(len(kinds)))]
+        L = Ls[int(rng.integers(len(Ls)))]
+        J2 = float(rng.uniform(-0.3, 0.3))
+        R = int(rng.integers(r_range[0], r_range[1] + 1))
+        out.append((kind, L, J2, R, ladder("log", t_range[0], t_range[1], R)))
+    return out

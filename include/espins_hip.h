@@ -85,6 +85,9 @@ int esp_plan_model(int N, int nb, const double *mag,
 /* Sweep kernel choice: 0 = automatic (LDS-resident when a replica fits in LDS), 1 = global-
  * memory colour phases, 2 = LDS-resident (EINVAL if the replica does not fit). */
 int esp_set_path(esp_ctx *ctx, int path);
+/* Philox stream of replica r is base + r (default 0); the exchange stream is
+ * 0xFFFFFFFF - base.  A batch gives chain k the base k << 16. */
+int esp_set_stream_base(esp_ctx *ctx, uint32_t base);
 /* colour[N]: 0-based colour of 1-based site i at colour[i-1]; *ncol colours. */
 int esp_get_coloring(esp_ctx *ctx, int *ncol, int *colour);
 /* Model facts: out[0]=N, out[1]=merged bond width, out[2]=colours, out[3]=1 if the
@@ -153,6 +156,48 @@ int esp_reset_stats(esp_ctx *ctx);
 int esp_sync(esp_ctx *ctx);
 /* The context's HIP stream (hipStream_t) for callers that time or order work on it. */
 void *esp_stream(esp_ctx *ctx);
+
+/* ---- batches of independent chains -------------------------------------------------- */
+/* Many PT chains (lattices of different kind and size, each with its own ladder) advanced
+ * together: one launch per colour index covers every chain, chains whose replica fits in
+ * LDS run in a parallel graph branch, and a round of every chain is one hipGraph launch.
+ * Chain k uses Philox stream base k << 16 (see esp_set_stream_base). */
+typedef struct esp_batch esp_batch;
+int esp_batch_create(esp_batch **b, int device);
+int esp_batch_destroy(esp_batch *b);
+const char *esp_batch_last_error(const esp_batch *b);
+/* Arguments as esp_set_model + esp_set_supercell + esp_set_replicas; *chain_id receives
+ * the chain's index.  Chains with identical bond structure share one colouring. */
+int esp_batch_add_chain(esp_batch *b, int N, int nb, const double *mag,
+                        int zJ, const int *j_cnt, const int *j_nbr, const double *j_val,
+                        int zB, const int *b_cnt, const int *b_nbr, const double *b_val,
+                        int zD, const int *d_cnt, const int *d_nbr, const double *d_vec,
+                        int have_si, const double *si_par, const double *si_axis,
+                        int have_field, const double *h_par, const double *h_axis,
+                        const int *L3, int R, const double *tems, const double *spins,
+                        int *chain_id);
+/* Per-chain kernel shape: path 0 auto / 1 global / 2 LDS, tps lanes per site. */
+int esp_batch_set_chain_config(esp_batch *b, int chain, int path, int tps);
+/* Upload every chain (after the last esp_batch_add_chain). */
+int esp_batch_commit(esp_batch *b);
+/* Autotuner: for every distinct chain shape, time the candidate kernel shapes (path x
+ * lanes per site) on this device over nsweeps sweeps and keep the fastest.  cache_path
+ * (may be NULL) is a text file of earlier decisions, read first and rewritten. */
+int esp_batch_autotune(esp_batch *b, int nsweeps, const char *cache_path);
+/* esp_run for every chain at once (same schedule for all chains). */
+int esp_batch_run(esp_batch *b, int n_rounds, int steps_per_round, int steps_measure,
+                  int do_measure, int do_exchange, int start_parity, int mode, double tilt,
+                  uint64_t seed, uint64_t sweep0, uint64_t step0, int use_graph);
+int esp_batch_num_chains(esp_batch *b);
+/* out[8]: N, R, colours, lanes per site, LDS path, bond width, kernel variant, LDS fits. */
+int esp_batch_info(esp_batch *b, int chain, int *out8);
+int esp_batch_get_stats(esp_batch *b, int chain, double *acc, int64_t *rejected, double *e_run,
+                        int64_t *nmeas);
+int esp_batch_get_map(esp_batch *b, int chain, int *slot_rep, int64_t *nswap);
+int esp_batch_get_spins(esp_batch *b, int chain, int by_slot, double *spins);
+int esp_batch_energy(esp_batch *b, int chain, double *E);
+int esp_batch_reset_stats(esp_batch *b);
+int esp_batch_sync(esp_batch *b);
 
 #ifdef __cplusplus
 }

@@ -229,3 +229,24 @@ def test_run_graph_is_deterministic_and_equals_stepwise(En, path):
         assert np.array_equal(eng.spins(), outs[0][0])
         np.testing.assert_allclose(st["acc"], outs[0][1], rtol=1e-13)
         assert np.array_equal(eng.pt_map()[0], outs[0][3][0])
+
+
+@pytest.mark.parametrize("steps,M", [(2, 1), (3, 2)])
+def test_lds_mid_launch_measurement_matches_global_path(En, steps, M):
+    """Measurements in the middle of an LDS-resident launch leave the trajectory unchanged:
+    the fused LDS run equals the global-memory path and the stepwise run bit for bit."""
+    m, tems = Lt.config_c2(L=4, R=8)
+    spins = Lt.random_spins(m.N, 8, 1, m.mag)
+    outs = []
+    for path in (2, 1):
+        with En.Engine() as eng:
+            eng.set_model(m)
+            eng.set_path(path)
+            eng.set_replicas(tems, spins)
+            eng.run(1, steps_per_round=steps, steps_measure=M, measure=True, exchange=False,
+                    seed=9, graph=False)
+            st = eng.stats()
+            outs.append((eng.spins(), st["rejected"], st["acc"]))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    np.testing.assert_allclose(outs[0][2], outs[1][2], rtol=1e-13)
