@@ -118,6 +118,90 @@ def cpu_baseline(name, tems, seconds):
                         f"{seconds:.0f} s per core ({upd} updates, {wall:.1f} s wall)"))
 
 
+# ---------------------------------------------------------------- batched workload --------
+def _cpu_worker_c3(args):
+    kind, L, J2, T, seconds, seed = args
+    from espins_amd import lattice as Lt
+    from oracle import oracle as O
+    m = Lt.cubic_model(kind, L, J=(1.0, J2), builder=O.build_table_fn)
+    o = O.Oracle(m)
+    s = Lt.random_spins(m.N, 1, seed)[0]
+    st = O.mt_state(seed)
+    e = o.energy(s)
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        e, _ = o.sweep_ref(s, 1.0 / T, 0, 0.125, st, 1, e)
+        n += 1
+    return n * m.N, time.perf_counter() - t0
+
+
+def run_batch(args):
+    """256 independent PT chains (SC/BCC/FCC, L in {8..48}, 16-32 replicas, J1 = 1, random J2)
+    advanced together; one step = one sweep + measurement + exchange of every chain."""
+    import multiprocessing as mp
+    from espins_amd import lattice as Lt
+    chains = Lt.config_c3(n_chains=args.chains, seed=7)
+    cpu = None
+    if not args.no_cpu:
+        ncores = max(1, min(len(os.sched_getaffinity(0)), 16))
+        jobs = [(k, L, J2, float(tems[len(tems) // 2]), args.cpu_seconds, 100 + i)
+                for i, (k, L, J2, R, tems) in enumerate(chains[:ncores])]
+        log(f"cpu_baseline: {len(jobs)} chains of the batch, {args.cpu_seconds:.0f} s per core ...")
+        with mp.get_context("fork").Pool(len(jobs)) as pool:
+            res = pool.map(_cpu_worker_c3, jobs)
+        upd = sum(r[0] for r in res)
+        cpu = dict(value=upd / max(r[1] for r in res), unit="updates/s", cores=len(jobs), kind="port",
+                   sample=(f"oracle reference-order Metropolis (MT19937, FP64) on the first {len(jobs)} "
+                           f"chains of the batch (one chain, one temperature per core), "
+                           f"{args.cpu_seconds:.0f} s per core"))
+        log(f"cpu_baseline: {cpu['value'] / 1e6:.2f} M updates/s on {len(jobs)} cores")
+    from espins_amd.engine import Batch
+    t0 = time.perf_counter()
+    b = Batch(0)
+    total = 0
+    for i, (kind, L, J2, R, tems) in enumerate(chains):
+        m = Lt.cubic_model_fast(kind, L, J=(1.0, J2))
+        b.add_chain(m, tems, Lt.random_spins(m.N, R, 1000 + i))
+        total += m.N * R
+    b.commit()
+    log(f"batch: {len(chains)} chains, {total} spin updates per sweep, set up in "
+        f"{time.perf_counter() - t0:.1f} s")
+    t1 = time.perf_counter()
+    b.autotune(nsweeps=2, cache=args.tune_cache)
+    shapes = {}
+    for k in range(len(chains)):
+        i = b.info(k)
+        shapes[(i["lds"], i["tps"])] = shapes.get((i["lds"], i["tps"]), 0) + 1
+    log(f"autotune: {time.perf_counter() - t1:.1f} s, (lds, lanes) -> chains {shapes}")
+    graph = not args.no_graph
+    b.run(args.warmup, measure=True, exchange=True, seed=7, graph=graph)
+    b.reset_stats()
+    b.sync()
+    t2 = time.perf_counter()
+    b.run(args.steps, measure=True, exchange=True, seed=7, sweep0=args.warmup, step0=args.warmup,
+          graph=graph)
+    b.sync()
+    dt = time.perf_counter() - t2
+    value = total * args.steps / dt
+    rej = sum(int(b.stats(k)["rejected"].sum()) for k in range(len(chains)))
+    log(f"timed: {args.steps} steps in {dt:.3f} s -> {value / 1e9:.3f} G updates/s, "
+        f"acceptance {1 - rej / (total * args.steps):.4f}")
+    out = {
+        "metric": METRIC, "value": value, "unit": "updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": (f"{len(chains)} independent PT chains, SC/BCC/FCC, L in "
+                                "{8,12,16,24,32,48}, 16-32 replicas on geometric 0.5-2.5 J ladders, "
+                                "J1=1, J2~U[-0.3,0.3], exchange every sweep"),
+                   "chains": len(chains), "updates_per_step": total, "graph": graph,
+                   "kernel_shapes": {f"lds={k[0]},lanes={k[1]}": v for k, v in shapes.items()}},
+        "roofline": None, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    b.close()
+
+
 # ---------------------------------------------------------------- main ---------------------
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
@@ -130,6 +214,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--chains", type=int, default=256, help="batch_c3: number of chains")
+    ap.add_argument("--tune-cache", default=None, help="batch_c3: autotuner cache file")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1 "
@@ -137,6 +223,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.config == "batch_c3":
+        return run_batch(args)
     m, tems, desc, seed = workload(args.config)
     cpu = None
     if not args.no_cpu:   # before any GPU call (forked workers must not inherit a GPU context)
