@@ -15,7 +15,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("ESPINS_HIP_LIB", os.path.join(_HERE, "libespins_hip.so"))
 
 ESP_OK, ESP_EINVAL, ESP_EHIP, ESP_ESTATE, ESP_ENOMEM, ESP_ETABLE = 0, -1, -2, -3, -4, -5
-MODE_RANDOM, MODE_CONST = 0, 1
+MODE_RANDOM, MODE_CONST, MODE_LEARNED = 0, 1, 2
+LEARN_NPARAMS, LEARN_RECORD = 2502, 16
 
 
 class EspError(RuntimeError):
@@ -72,9 +73,25 @@ _SIGS = {
     "esp_get_stats": (_i, [_p, _dp, _i64p, _dp, _i64p]),
     "esp_get_map": (_i, [_p, _ip, _i64p]),
     "esp_reset_stats": (_i, [_p]),
+    "esp_set_trace": (_i, [_p, _i]),
+    "esp_get_trace": (_i, [_p, _dp, _ip]),
     "esp_sync": (_i, [_p]),
     "esp_stream": (_p, [_p]),
 }
+
+_fp = C.POINTER(C.c_float)
+_SIGS.update({
+    "esp_learn_set_weights": (_i, [_p, _fp, _i]),
+    "esp_learn_get_weights": (_i, [_p, _fp, _i]),
+    "esp_learn_collect": (_i, [_p, _i, _u64]),
+    "esp_learn_clear": (_i, [_p]),
+    "esp_learn_num_records": (_i, [_p, _ip]),
+    "esp_learn_get_records": (_i, [_p, _i, _i, _fp]),
+    "esp_learn_put_records": (_i, [_p, _fp, _i]),
+    "esp_learn_grad": (_i, [_p, _i, _i, _fp, _dp]),
+    "esp_learn_train": (_i, [_p, _i, _i, C.c_float, _u64, _dp]),
+    "esp_learn_eval": (_i, [_p, _i, _i, _dp]),
+})
 
 _SIGS.update({
     "esp_batch_create": (_i, [C.POINTER(_p), _i]),
@@ -138,3 +155,7 @@ def f64(a):
 
 def i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)

@@ -25,127 +25,9 @@ namespace {
 constexpr int kRedBlocks = 64;    // reduce blocks per replica (upper bound)
 }  // namespace
 
-// ===================================================================================
-// Context
-// ===================================================================================
-struct esp_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  size_t lds_max = 65536;
-  int Lhint[3] = {0, 0, 0};       // supercell size for the periodic colouring
-  int path = 0;                   // 0 auto, 1 global-memory phases, 2 LDS-resident
-  uint32_t stream_base = 0;       // Philox stream of replica r: stream_base + r
-  bool lds_fits = false;
-  // model
-  esp::HostModel hm;
-  bool have_model = false;
-  int *d_nbr = nullptr, *d_dev2ref = nullptr, *d_ref2dev = nullptr, *d_col_off = nullptr;
-  double *d_cJ = nullptr, *d_cB = nullptr, *d_cD = nullptr, *d_ctab = nullptr;
-  double *d_mag = nullptr, *d_si4 = nullptr, *d_h4 = nullptr;
-  uint8_t *d_basis = nullptr;
-  bool use_lds = false;
-  size_t lds_bytes = 0, ctab_bytes = 0;
-  // replicas
-  int R = 0;
-  double *d_sp = nullptr;         // [R][3][N]
-  double *d_w = nullptr;          // [R][N]
-  int tps = 1;                    // lanes per site in the sweep kernels
-  int Ns = 0;                     // replica stride (> N; index N is the zero spin)
-  int tps_user = 0;               // 0 = automatic
-  double *d_beta = nullptr, *d_ebase = nullptr, *d_erun = nullptr, *d_acc = nullptr;
-  double *d_partial = nullptr;
-  int red_blocks = 1;
-  int *d_slot_rep = nullptr, *d_rep_slot = nullptr;
-  unsigned long long *d_rej = nullptr, *d_nswap = nullptr, *d_nmeas = nullptr;
-  Clock *d_clock = nullptr;
-  std::vector<double> tems;
-  // host-fed staging
-  double *d_prop = nullptr, *d_unif = nullptr;
-  uint8_t *d_accd = nullptr;
-  // graph cache
-  hipGraphExec_t gexec = nullptr;
-  std::string gkey;
-};
+#include "esp_ctx.h"
 
 namespace {
-
-int fail(esp_ctx *c, int code, const std::string &msg) {
-  if (c) c->err = msg;
-  return code;
-}
-
-#define HIPCHK(ctx, expr)                                                              \
-  do {                                                                                 \
-    hipError_t e_ = (expr);                                                            \
-    if (e_ != hipSuccess)                                                              \
-      return fail(ctx, ESP_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
-  } while (0)
-
-template <class T>
-void dfree(T *&p) {
-  if (p) (void)hipFree((void *)p);
-  p = nullptr;
-}
-
-template <class T>
-int dalloc(esp_ctx *c, T *&p, size_t n) {
-  dfree(p);
-  if (n == 0) n = 1;
-  if (hipMalloc((void **)&p, n * sizeof(T)) != hipSuccess) {
-    p = nullptr;
-    return fail(c, ESP_ENOMEM, "hipMalloc failed");
-  }
-  return ESP_OK;
-}
-
-template <class T>
-int upload(esp_ctx *c, T *&p, const std::vector<T> &v) {
-  int rc = dalloc(c, p, v.size());
-  if (rc) return rc;
-  if (!v.empty())
-    HIPCHK(c, hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
-  return ESP_OK;
-}
-
-void drop_graph(esp_ctx *c) {
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  c->gexec = nullptr;
-  c->gkey.clear();
-}
-
-void free_replicas(esp_ctx *c) {
-  dfree(c->d_sp); dfree(c->d_w); dfree(c->d_beta); dfree(c->d_ebase); dfree(c->d_erun); dfree(c->d_acc);
-  dfree(c->d_partial); dfree(c->d_slot_rep); dfree(c->d_rep_slot); dfree(c->d_rej);
-  dfree(c->d_nswap); dfree(c->d_nmeas); dfree(c->d_prop); dfree(c->d_unif); dfree(c->d_accd);
-  c->R = 0;
-  drop_graph(c);
-}
-
-void free_model(esp_ctx *c) {
-  dfree(c->d_nbr); dfree(c->d_dev2ref); dfree(c->d_ref2dev); dfree(c->d_col_off);
-  dfree(c->d_cJ); dfree(c->d_cB); dfree(c->d_cD); dfree(c->d_ctab); dfree(c->d_mag);
-  dfree(c->d_si4); dfree(c->d_h4); dfree(c->d_basis);
-  c->have_model = false;
-  drop_graph(c);
-}
-
-KArgs base_args(esp_ctx *c) {
-  KArgs a{};
-  const esp::HostModel &m = c->hm;
-  a.sp = c->d_sp;
-  a.w = c->d_w;
-  a.tps = c->tps;
-  a.nbr = c->d_nbr;
-  a.cJ = c->d_cJ; a.cB = c->d_cB; a.cD = c->d_cD; a.ctab = c->d_ctab;
-  a.mag = c->d_mag; a.basis = c->d_basis; a.dev2ref = c->d_dev2ref; a.col_off = c->d_col_off;
-  a.si4 = c->d_si4; a.h4 = c->d_h4;
-  a.beta_slot = c->d_beta; a.rep_slot = c->d_rep_slot; a.rej = c->d_rej; a.clock = c->d_clock;
-  a.N = m.N; a.Ns = c->Ns; a.nb = m.nb; a.zmax = m.zmax; a.R = c->R; a.ncol = m.ncol;
-  a.has_si = m.has_si; a.has_h = m.has_h;
-  a.stream_base = c->stream_base;
-  return a;
-}
 
 // kernel tables: [mode][cmp][hb][hd][self]
 using PhaseFn = void (*)(KArgs);
@@ -196,6 +78,7 @@ int check_ready(esp_ctx *c, bool need_replicas) {
 
 // Enqueue `nsweeps` sweeps starting at clock->sweep + off0 (global-memory path).
 int enqueue_phase_sweeps(esp_ctx *c, int mode, double tilt, uint64_t seed, int nsweeps, int off0) {
+  if (mode == ESP_MODE_LEARNED) return esp::learn_enqueue_sweeps(c, seed, nsweeps, off0);
   const esp::HostModel &m = c->hm;
   KArgs a = base_args(c);
   a.tilt = tilt;
@@ -247,6 +130,7 @@ int enqueue_finalize(esp_ctx *c, bool use_partials, bool measure, bool exchange,
   f.nswap = c->d_nswap; f.clock = c->d_clock;
   f.measure = measure; f.exchange = exchange; f.advance_sweeps = advance;
   f.key0 = (uint32_t)seed; f.key1 = (uint32_t)(seed >> 32); f.stream_base = c->stream_base;
+  f.trace = c->d_trace; f.trace_cap = c->trace_cap;
   const int nt = std::min(1024, std::max(64, ((c->R + 63) / 64) * 64));
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(nt), 0, c->stream, f);
   HIPCHK(c, hipGetLastError());
@@ -287,11 +171,17 @@ int rebase(esp_ctx *c) {
   return ESP_OK;
 }
 
+// The LDS-resident kernel serves the plain proposals when the replica fits and no trace
+// is recorded; learned proposals and traced runs use the global-memory path.
+bool lds_sweeps(const esp_ctx *c, int mode) {
+  return c->use_lds && mode != ESP_MODE_LEARNED && !c->d_trace;
+}
+
 // One MC "round": steps x (M sweeps [+ measure]) [+ exchange]; advances the clock.
 int enqueue_round(esp_ctx *c, int steps, int M, bool measure, bool exchange, int mode,
                   double tilt, uint64_t seed) {
   int rc;
-  if (c->use_lds) {
+  if (lds_sweeps(c, mode)) {
     if (measure) {
       rc = enqueue_lds_sweeps(c, mode, tilt, seed, steps * M, M);
       if (rc) return rc;
@@ -365,6 +255,7 @@ int esp_destroy(esp_ctx *c) {
   (void)hipStreamSynchronize(c->stream);
   free_replicas(c);
   free_model(c);
+  esp::learn_free(c);
   dfree(c->d_clock);
   (void)hipStreamDestroy(c->stream);
   delete c;
@@ -633,11 +524,11 @@ int esp_energy(esp_ctx *c, double *E) {
 int esp_sweep(esp_ctx *c, int nsweeps, int mode, double tilt, uint64_t seed, uint64_t sweep0) {
   int rc = check_ready(c, true);
   if (rc) return rc;
-  if (nsweeps < 0 || (mode != ESP_MODE_RANDOM && mode != ESP_MODE_CONST))
+  if (nsweeps < 0 || (mode != ESP_MODE_RANDOM && mode != ESP_MODE_CONST && mode != ESP_MODE_LEARNED))
     return fail(c, ESP_EINVAL, "bad nsweeps / mode");
   if (nsweeps == 0) return ESP_OK;
   if ((rc = set_clock(c, sweep0, 0, 1))) return rc;
-  if (c->use_lds) rc = enqueue_lds_sweeps(c, mode, tilt, seed, nsweeps, 0);
+  if (lds_sweeps(c, mode)) rc = enqueue_lds_sweeps(c, mode, tilt, seed, nsweeps, 0);
   else rc = enqueue_phase_sweeps(c, mode, tilt, seed, nsweeps, 0);
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -703,7 +594,8 @@ int esp_run(esp_ctx *c, int n_rounds, int steps_per_round, int steps_measure, in
   int rc = check_ready(c, true);
   if (rc) return rc;
   if (n_rounds < 0 || steps_per_round < 1 || steps_measure < 1 ||
-      (start_parity != 1 && start_parity != 2) || (mode != ESP_MODE_RANDOM && mode != ESP_MODE_CONST))
+      (start_parity != 1 && start_parity != 2) ||
+      (mode != ESP_MODE_RANDOM && mode != ESP_MODE_CONST && mode != ESP_MODE_LEARNED))
     return fail(c, ESP_EINVAL, "bad esp_run arguments");
   if (n_rounds == 0) return ESP_OK;
   if ((rc = set_clock(c, sweep0, step0, start_parity))) return rc;
@@ -829,6 +721,38 @@ int esp_get_map(esp_ctx *c, int *slot_rep, int64_t *nswap) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (slot_rep)
     for (int t = 0; t < R; ++t) slot_rep[t] = sr[t] + 1;
+  return ESP_OK;
+}
+
+int esp_set_trace(esp_ctx *c, int capacity) {
+  int rc = check_ready(c, true);
+  if (rc) return rc;
+  if (capacity < 0 || (int64_t)capacity * c->R > ((int64_t)1 << 28))
+    return fail(c, ESP_EINVAL, "esp_set_trace: capacity out of range");
+  dfree(c->d_trace);
+  c->trace_cap = 0;
+  if (capacity > 0) {
+    if ((rc = dalloc(c, c->d_trace, (size_t)capacity * c->R * 2))) return rc;
+    c->trace_cap = capacity;
+  }
+  drop_graph(c);
+  return ESP_OK;
+}
+
+int esp_get_trace(esp_ctx *c, double *trace, int *n) {
+  int rc = check_ready(c, true);
+  if (rc) return rc;
+  if (!n) return fail(c, ESP_EINVAL, "esp_get_trace: null count");
+  unsigned long long nm = 0;
+  HIPCHK(c, hipMemcpyAsync(&nm, c->d_nmeas, sizeof nm, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int k = (int)std::min<unsigned long long>(nm, (unsigned long long)c->trace_cap);
+  *n = k;
+  if (k > 0 && trace) {
+    HIPCHK(c, hipMemcpyAsync(trace, c->d_trace, sizeof(double) * 2 * (size_t)k * c->R, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
   return ESP_OK;
 }
 

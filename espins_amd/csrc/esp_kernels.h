@@ -13,6 +13,13 @@
 #include "esp_device.h"
 
 namespace espk {
+struct Clock {                   // device-side schedule counters (graph-replayable)
+  unsigned long long sweep;      // Philox sweep counter of the next sweep
+  unsigned long long step;       // exchange step counter
+  int parity;                    // next exchange start (1 or 2)
+  int pad;
+};
+
 namespace {  // internal linkage: each translation unit that launches these owns its copies
 using namespace espd;
 
@@ -22,13 +29,6 @@ constexpr int kChunk = 8;        // partner gathers issued together per lane
 // Dynamic LDS of the LDS-resident kernel: [N+1 spins as double4][coefficient table]
 // [kLdsRed doubles of reduction scratch]; every byte of LDS it uses is in that one region.
 constexpr int kLdsRed = 4 * (512 / 64);
-
-struct Clock {                   // device-side schedule counters (graph-replayable)
-  unsigned long long sweep;      // Philox sweep counter of the next sweep
-  unsigned long long step;       // exchange step counter
-  int parity;                    // next exchange start (1 or 2)
-  int pad;
-};
 
 struct KArgs {
   double *sp;                    // [R][3][Ns]
@@ -426,6 +426,8 @@ struct FinArgs {
   Clock *clock;
   int measure, exchange, advance_sweeps;
   uint32_t key0, key1, stream_base;
+  double *trace;                 // [trace_cap][R][2] by slot: E, |m| per measurement, or null
+  int trace_cap;
 };
 
 // Running energies / measurement, then the replica exchange (pt_swap, pt.F90:1235-1288)
@@ -448,6 +450,12 @@ __device__ __forceinline__ void finalize_body(const FinArgs &f, int start, unsig
         const double e2 = e * e;
         acc[0] += e; acc[1] += e2; acc[2] += e2 * e2;
         acc[3] += sqrt(m2); acc[4] += m2; acc[5] += m2 * m2;
+        const unsigned long long k = f.nmeas[0];   // incremented after the barrier below
+        if (f.trace && k < (unsigned long long)f.trace_cap) {
+          double *t = f.trace + 2 * ((size_t)k * f.R + f.rep_slot[r]);
+          t[0] = e;
+          t[1] = sqrt(m2);
+        }
       }
     }
   __syncthreads();

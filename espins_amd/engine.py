@@ -26,7 +26,8 @@ import numpy as np
 from . import _lib as E
 from .lattice import Model
 
-MODES = {"random": E.MODE_RANDOM, "rand": E.MODE_RANDOM, "const": E.MODE_CONST}
+MODES = {"random": E.MODE_RANDOM, "rand": E.MODE_RANDOM, "const": E.MODE_CONST,
+         "learned": E.MODE_LEARNED}
 
 
 def _mode(m):
@@ -204,11 +205,76 @@ class Engine:
     def reset_stats(self):
         self._chk(E.lib.esp_reset_stats(self._h))
 
+    def set_trace(self, capacity):
+        """Record E and |m| per slot at every measurement (0 = off)."""
+        self._chk(E.lib.esp_set_trace(self._h, int(capacity)))
+        return self
+
+    def trace(self):
+        """[n][R][2] by slot: E and |m| of each measurement since the last reset_stats."""
+        n = C.c_int()
+        self._chk(E.lib.esp_get_trace(self._h, None, C.byref(n)))
+        out = np.zeros((n.value, self.R, 2))
+        if n.value:
+            self._chk(E.lib.esp_get_trace(self._h, E.ptr(out, C.c_double), C.byref(n)))
+        return out
+
     def sync(self):
         self._chk(E.lib.esp_sync(self._h))
 
     def stream(self):
         return E.lib.esp_stream(self._h)
+
+    # ---- learned proposal -------------------------------------------------------------
+    def learn_set_weights(self, w):
+        w = E.f32(w)
+        self._chk(E.lib.esp_learn_set_weights(self._h, E.ptr(w, C.c_float), w.size))
+        return self
+
+    def learn_weights(self):
+        w = np.zeros(E.LEARN_NPARAMS, np.float32)
+        self._chk(E.lib.esp_learn_get_weights(self._h, E.ptr(w, C.c_float), w.size))
+        return w
+
+    def learn_collect(self, n, seed=1):
+        self._chk(E.lib.esp_learn_collect(self._h, int(n), int(seed)))
+
+    def learn_clear(self):
+        self._chk(E.lib.esp_learn_clear(self._h))
+
+    def learn_num_records(self):
+        n = C.c_int()
+        self._chk(E.lib.esp_learn_num_records(self._h, C.byref(n)))
+        return n.value
+
+    def learn_records(self, first=0, n=None):
+        n = self.learn_num_records() - first if n is None else n
+        out = np.zeros((n, E.LEARN_RECORD), np.float32)
+        self._chk(E.lib.esp_learn_get_records(self._h, int(first), int(n), E.ptr(out, C.c_float)))
+        return out
+
+    def learn_put_records(self, recs):
+        recs = E.f32(recs).reshape(-1, E.LEARN_RECORD)
+        self._chk(E.lib.esp_learn_put_records(self._h, E.ptr(recs, C.c_float), recs.shape[0]))
+
+    def learn_grad(self, first=0, n=None):
+        n = self.learn_num_records() - first if n is None else n
+        g = np.zeros(E.LEARN_NPARAMS, np.float32)
+        loss = C.c_double()
+        self._chk(E.lib.esp_learn_grad(self._h, int(first), int(n), E.ptr(g, C.c_float), C.byref(loss)))
+        return g, loss.value
+
+    def learn_train(self, epochs=1, batch=1024, lr=1e-3, seed=1):
+        out = np.zeros(max(epochs, 1))
+        self._chk(E.lib.esp_learn_train(self._h, int(epochs), int(batch), float(lr), int(seed),
+                                        E.ptr(out, C.c_double)))
+        return out[:epochs]
+
+    def learn_eval(self, first=0, n=None):
+        n = self.learn_num_records() - first if n is None else n
+        out = np.zeros((n, 10))
+        self._chk(E.lib.esp_learn_eval(self._h, int(first), int(n), E.ptr(out, C.c_double)))
+        return out
 
 
 # ---- estimators --------------------------------------------------------------------------

@@ -34,6 +34,10 @@ extern "C" {
 
 #define ESP_MODE_RANDOM 0 /* mcarlo_mode = random (get_quant.F90:80-93)          */
 #define ESP_MODE_CONST  1 /* mcarlo_mode = const  (get_quant.F90:94-161)         */
+#define ESP_MODE_LEARNED 2 /* learned vMF-mixture proposal, Metropolis-Hastings  */
+
+#define ESP_LEARN_NPARAMS 2502 /* MLP 5 -> 32 -> 32 -> 32 -> 6 (weights + biases)   */
+#define ESP_LEARN_RECORD  16   /* floats per local-conditional record               */
 
 typedef struct esp_ctx esp_ctx;
 
@@ -152,10 +156,48 @@ int esp_get_stats(esp_ctx *ctx, double *acc, int64_t *rejected, double *e_run, i
 int esp_get_map(esp_ctx *ctx, int *slot_rep, int64_t *nswap);
 /* Zero acc, rejected, nswap and the measurement count (phase boundaries, pt.F90:650). */
 int esp_reset_stats(esp_ctx *ctx);
+/* Record E and |m| of every slot at each measurement (capacity measurements, counted by
+ * nmeas from the last esp_reset_stats; 0 switches the trace off).  While the trace is on,
+ * sweeps use the global-memory path.  trace[n][R][2] by slot; *n = measurements stored. */
+int esp_set_trace(esp_ctx *ctx, int capacity);
+int esp_get_trace(esp_ctx *ctx, double *trace, int *n);
 /* Wait for the context's stream. */
 int esp_sync(esp_ctx *ctx);
 /* The context's HIP stream (hipStream_t) for callers that time or order work on it. */
 void *esp_stream(esp_ctx *ctx);
+
+/* ---- learned proposal (no counterpart in the reference) --------------------------------
+ * mode ESP_MODE_LEARNED in esp_sweep / esp_run draws each new spin from a two-component
+ * von Mises-Fisher mixture whose parameters an MLP computes from the site's environment
+ * (local field, temperature, biquadratic and single-ion projections), plus a fixed
+ * uniform-sphere component of weight 1/32, and accepts with
+ * min(1, exp(-beta dE) q(S_old) / q(S_new)).  The proposal depends on the neighbours only,
+ * so detailed balance holds for any weights.  Global-memory path, one lane per site;
+ * models with self bonds are refused.
+ * Record layout (ESP_LEARN_RECORD floats): features x[5], kappa0 = beta |m| |G|,
+ * ghat[3] (field direction), axis[3] (single-ion axis, else z), s[3] (spin direction),
+ * replica * N + site (1 float; exact below 2^24).
+ * Parameter layout: W1[32][5] b1[32] W2[32][32] b2[32] W3[32][32] b3[32] W4[6][32] b4[6];
+ * outputs per component k: log kappa scale, weight logit, tilt (pi/2 tanh) toward the axis. */
+/* FP32 master weights (copied to BF16 for the sweep); resets the Adam state. */
+int esp_learn_set_weights(esp_ctx *ctx, const float *w, int nparams);
+int esp_learn_get_weights(esp_ctx *ctx, float *w, int nparams);
+/* Append n records of the current configurations: (replica, site) drawn from Philox key seed. */
+int esp_learn_collect(esp_ctx *ctx, int n, uint64_t seed);
+int esp_learn_clear(esp_ctx *ctx);
+int esp_learn_num_records(esp_ctx *ctx, int *n);
+int esp_learn_get_records(esp_ctx *ctx, int first, int n, float *recs);
+/* Replace the record set with n host records. */
+int esp_learn_put_records(esp_ctx *ctx, const float *recs, int n);
+/* Gradient of the mean negative log-likelihood -log q(s | env) over records
+ * [first, first+n) at the current master weights: grad[ESP_LEARN_NPARAMS], *loss. */
+int esp_learn_grad(esp_ctx *ctx, int first, int n, float *grad, double *loss);
+/* Adam (lr, beta 0.9 / 0.999, eps 1e-8) over the record set: epochs x shuffled minibatches
+ * of `batch`; epoch_loss[epochs] (may be NULL) = mean loss seen in each epoch. */
+int esp_learn_train(esp_ctx *ctx, int epochs, int batch, float lr, uint64_t seed,
+                    double *epoch_loss);
+/* Per record [n][10]: the six outputs, kappa_0, kappa_1, log q(s | env), w_0. */
+int esp_learn_eval(esp_ctx *ctx, int first, int n, double *out);
 
 /* ---- batches of independent chains -------------------------------------------------- */
 /* Many PT chains (lattices of different kind and size, each with its own ladder) advanced
