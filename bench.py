@@ -202,6 +202,115 @@ def run_batch(args):
     b.close()
 
 
+# ---------------------------------------------------------------- learned proposal --------
+def run_learned(args):
+    """Frustrated FCC 32^3 PT (N = 131072, 64 replicas): train the learned proposal on local
+    conditionals of equilibrated plain chains, then compare effective samples/s of E of the
+    learned chain with plain Metropolis ('random' and 'const' proposals) on the same schedule
+    (one sweep + measurement + exchange per step) from the same start."""
+    import numpy as np
+    from espins_amd import lattice as Lt
+    from espins_amd import learn as Ln
+    m, tems, desc, seed = workload("pt_fcc32")
+    cpu = None
+    if not args.no_cpu:
+        log(f"cpu_baseline: {args.cpu_seconds:.0f} s per core ...")
+        cpu = cpu_baseline("pt_fcc32", tems, args.cpu_seconds)
+        log(f"cpu_baseline: {cpu['value'] / 1e6:.2f} M updates/s on {cpu['cores']} cores")
+    from espins_amd.engine import Engine
+    R = len(tems)
+    eng = Engine(0)
+    eng.set_model(m)
+    # Ordered start (four-sublattice tetrahedral state, E/N = -2.25 J, the lowest of the
+    # simple orders of this Hamiltonian): the traces then measure equilibrium autocorrelation
+    # at the cold end instead of domain coarsening from a random start.
+    init = (Lt.sublattice_spins(m, Lt.FCC_TETRAHEDRAL, R) if args.start == "ordered"
+            else Lt.random_spins(m.N, R, seed, m.mag))
+    eng.set_replicas(tems, init)
+    graph = not args.no_graph
+
+    def pt(n, mode, sweep0, tilt=0.0):
+        eng.run(n, steps_per_round=1, steps_measure=1, measure=True, exchange=True,
+                start_parity=1 + (sweep0 % 2), mode=mode, tilt=tilt, seed=seed, sweep0=sweep0,
+                step0=sweep0, graph=graph)
+
+    t0 = time.perf_counter()
+    pt(args.warmup, "random", 0)
+    eng.sync()
+    log(f"equilibration: {args.warmup} PT steps, {time.perf_counter() - t0:.1f} s")
+    t1 = time.perf_counter()
+    losses = Ln.train_proposal(eng, n_records=args.records, collect_every=2,
+                               per_collect=max(1, args.records // 16), epochs=args.epochs,
+                               batch=4096, lr=2e-3, seed=seed, mode="random", sweep0=args.warmup)
+    t_train = time.perf_counter() - t1
+    log(f"training: {args.records} records, {args.epochs} epochs, loss {losses[0]:.4f} -> "
+        f"{losses[-1]:.4f}, {t_train:.1f} s (collection sweeps included)")
+    start = eng.spins(by_slot=True)
+    sweep0 = args.warmup + 64
+    res = {}
+    for mode, tilt in (("random", 0.0), ("const", args.tilt), ("learned", 0.0)):
+        eng.set_replicas(tems, start)
+        pt(args.ess_warmup, mode, sweep0, tilt)
+        eng.reset_stats()
+        eng.set_trace(args.steps)
+        eng.sync()
+        t2 = time.perf_counter()
+        pt(args.steps, mode, sweep0 + args.ess_warmup, tilt)
+        eng.sync()
+        dt = time.perf_counter() - t2
+        tr = eng.trace()[:, :, 0]
+        st = eng.stats()
+        eng.set_trace(0)
+        ess, tau = Ln.ess_per_slot(tr)
+        acc = 1.0 - st["rejected"] / float(m.N * args.steps)
+        h = args.steps // 2       # drift: first-half minus second-half mean, in standard errors
+        se = np.sqrt(2 * tau / h) * tr.std(0)
+        drift = (tr[:h].mean(0) - tr[h:].mean(0)) / np.maximum(se * np.sqrt(2), 1e-300)
+        res[mode] = dict(ess_s=ess / dt, tau=tau, dt=dt, acc=acc, e=tr.mean(0), drift=drift)
+        log(f"{mode:8s}: {args.steps} steps in {dt:.2f} s ({m.N * R * args.steps / dt / 1e9:.3f} G updates/s), "
+            f"acceptance {acc.min():.4f}-{acc.max():.4f}, tau_int(E) median {np.median(tau):.1f} "
+            f"max {tau.max():.1f} sweeps, ESS/s total {ess.sum() / dt:.1f}, "
+            f"half-to-half drift max {np.abs(drift).max():.1f} sigma")
+        log("   tau by slot (cold -> hot): " + " ".join(f"{x:.0f}" for x in tau[::4]))
+    lo = slice(0, R // 4)                       # coldest quarter of the ladder
+    ratio = {k: res["learned"]["ess_s"] / res[k]["ess_s"] for k in ("random", "const")}
+    reliable = int(np.sum(res["random"]["tau"] * 50 < args.steps))
+    tm = eng.time_sweeps(20, mode="learned", seed=seed + 1, sweep0=10 ** 9)
+    B = bytes_per_update(m)
+    ms_launch = tm["kernel_ms"] / tm["nlaunch"]
+    achieved = tm["updates_per_launch"] * B / (ms_launch * 1e-3) / 1e9
+    log(f"k_phase_learned: mean {ms_launch * 1e3:.1f} us per launch, "
+        f"{tm['updates_per_launch']} updates/launch -> {achieved:.0f} GB/s algorithmic")
+    L = res["learned"]
+    out = {
+        "metric": METRIC, "value": float(L["ess_s"].sum()), "unit": "effective samples of E/s (sum over slots)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": L["dt"] / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": desc + ", learned vMF-mixture proposal vs plain Metropolis", "N": m.N,
+                   "replicas": R, "records": args.records, "epochs": args.epochs, "graph": graph,
+                   "const_tilt": args.tilt, "ess_warmup": args.ess_warmup, "start": args.start},
+        "ess_ratio_vs_random": {"median": float(np.median(ratio["random"])),
+                                "coldest_quarter": float(np.median(ratio["random"][lo])),
+                                "min": float(ratio["random"].min())},
+        "ess_ratio_vs_const": {"median": float(np.median(ratio["const"])),
+                               "coldest_quarter": float(np.median(ratio["const"][lo])),
+                               "min": float(ratio["const"].min())},
+        "tau_int_E": {k: [float(x) for x in v["tau"]] for k, v in res.items()},
+        "random_tau_reliable_slots": reliable,
+        "drift_sigma_max": {k: float(np.abs(v["drift"]).max()) for k, v in res.items()},
+        "updates_per_s": {k: m.N * R * args.steps / v["dt"] for k, v in res.items()},
+        "acceptance_mean": {k: float(v["acc"].mean()) for k, v in res.items()},
+        "train": {"loss_first": float(losses[0]), "loss_last": float(losses[-1]), "seconds": t_train},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_phase_learned",
+                     "bytes_per_update": B, "updates_per_launch": tm["updates_per_launch"],
+                     "mean_launch_us": ms_launch * 1e3},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
 # ---------------------------------------------------------------- main ---------------------
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
@@ -216,6 +325,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chains", type=int, default=256, help="batch_c3: number of chains")
     ap.add_argument("--tune-cache", default=None, help="batch_c3: autotuner cache file")
+    ap.add_argument("--records", type=int, default=1 << 20, help="learned_fcc32: training records")
+    ap.add_argument("--epochs", type=int, default=4, help="learned_fcc32: training epochs")
+    ap.add_argument("--tilt", type=float, default=0.125, help="learned_fcc32: 'const' cone tilt")
+    ap.add_argument("--ess-warmup", type=int, default=500, help="learned_fcc32: sweeps before the trace")
+    ap.add_argument("--start", default="ordered", choices=("ordered", "random"),
+                    help="learned_fcc32: initial spins")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1 "
@@ -225,6 +340,8 @@ def main():
 
     if args.config == "batch_c3":
         return run_batch(args)
+    if args.config == "learned_fcc32":
+        return run_learned(args)
     m, tems, desc, seed = workload(args.config)
     cpu = None
     if not args.no_cpu:   # before any GPU call (forked workers must not inherit a GPU context)

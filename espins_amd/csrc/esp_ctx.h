@@ -154,6 +154,9 @@ KArgs base_args(esp_ctx *c) {
 
 namespace esp {
 // Learned-proposal state (esp_learn.hip).
-int learn_enqueue_sweeps(esp_ctx *c, uint64_t seed, int nsweeps, int off0);
+// ev (optional): 2 events per launch, recorded by the dispatch packets (timing); *nlaunch
+// receives the number of launches.
+int learn_enqueue_sweeps(esp_ctx *c, uint64_t seed, int nsweeps, int off0, hipEvent_t *ev = nullptr,
+                         int *nlaunch = nullptr);
 void learn_free(esp_ctx *c);
 }  // namespace esp

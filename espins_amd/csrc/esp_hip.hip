@@ -635,16 +635,23 @@ int esp_time_sweeps(esp_ctx *c, int nsweeps, int mode, double tilt, uint64_t see
                     double *kernel_ms, int *nlaunch, double *wall_ms, int64_t *updates_per_launch) {
   int rc = check_ready(c, true);
   if (rc) return rc;
-  if (nsweeps < 1 || (mode != ESP_MODE_RANDOM && mode != ESP_MODE_CONST) || !kernel_ms || !nlaunch)
+  if (nsweeps < 1 || (mode != ESP_MODE_RANDOM && mode != ESP_MODE_CONST && mode != ESP_MODE_LEARNED) ||
+      !kernel_ms || !nlaunch)
     return fail(c, ESP_EINVAL, "bad esp_time_sweeps arguments");
   if ((rc = set_clock(c, sweep0, 0, 1))) return rc;
   const esp::HostModel &m = c->hm;
+  const bool lds = lds_sweeps(c, mode);
   // one event pair around every sweep-kernel launch, on the context's stream
-  const int per = c->use_lds ? 1 : nsweeps * m.ncol;
+  const int per = lds ? 1 : nsweeps * m.ncol;
   std::vector<hipEvent_t> ev(2 * per, nullptr);
   for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
   int k = 0;
-  if (c->use_lds) {
+  if (mode == ESP_MODE_LEARNED) {
+    if ((rc = esp::learn_enqueue_sweeps(c, seed, nsweeps, 0, ev.data(), &k))) {
+      for (auto &e : ev) (void)hipEventDestroy(e);
+      return rc;
+    }
+  } else if (lds) {
     KArgs a = base_args(c);
     a.tilt = tilt;
     a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32);
@@ -689,7 +696,7 @@ int esp_time_sweeps(esp_ctx *c, int nsweeps, int mode, double tilt, uint64_t see
   *nlaunch = k;
   if (wall_ms) *wall_ms = wall;
   if (updates_per_launch)
-    *updates_per_launch = c->use_lds ? (int64_t)m.N * c->R * nsweeps : (int64_t)(m.N / m.ncol) * c->R;
+    *updates_per_launch = lds ? (int64_t)m.N * c->R * nsweeps : (int64_t)m.N * c->R * nsweeps / k;
   return rc;
 }
 

@@ -15,6 +15,7 @@
 // concentration kappa_k = beta|G| exp(s_k), weight logit, tilt t_k = (pi/2) tanh(.) of the
 // mean direction from G-hat toward the nearer pole of the anisotropy axis.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -610,7 +611,7 @@ int enqueue_grad(esp_ctx *c, const int *perm, int off, int n, int k) {
 namespace esp {
 
 // nsweeps sweeps of learned proposals from clock->sweep + off0 (global-memory path).
-int learn_enqueue_sweeps(esp_ctx *c, uint64_t seed, int nsweeps, int off0) {
+int learn_enqueue_sweeps(esp_ctx *c, uint64_t seed, int nsweeps, int off0, hipEvent_t *ev, int *nlaunch) {
   int rc = learn_ready(c, true);
   if (rc) return rc;
   if (!c->have_learn) return fail(c, ESP_ESTATE, "no proposal weights: call esp_learn_set_weights first");
@@ -620,6 +621,7 @@ int learn_enqueue_sweeps(esp_ctx *c, uint64_t seed, int nsweeps, int off0) {
   a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32);
   LearnFn fn = learn_fn(m);
   const size_t shm = (m.compressed ? c->ctab_bytes : 0) + kNP * sizeof(float);
+  int k = 0;
   for (int s = 0; s < nsweeps; ++s)
     for (int col = 0; col < m.ncol; ++col) {
       a.c_begin = m.col_off[col];
@@ -628,9 +630,15 @@ int learn_enqueue_sweeps(esp_ctx *c, uint64_t seed, int nsweeps, int off0) {
       a.nblk = (a.c_count + kBlock - 1) / kBlock;
       a.xcd_remap = (c->R % 8 == 0) ? 1 : 0;
       a.sweep_off = off0 + s;
-      hipLaunchKernelGGL(fn, dim3(a.nblk * c->R), dim3(kBlock), shm, c->stream, a, (const uint16_t *)c->d_lbf);
+      if (ev)
+        hipExtLaunchKernelGGL(fn, dim3(a.nblk * c->R), dim3(kBlock), shm, c->stream, ev[2 * k], ev[2 * k + 1], 0,
+                              a, (const uint16_t *)c->d_lbf);
+      else
+        hipLaunchKernelGGL(fn, dim3(a.nblk * c->R), dim3(kBlock), shm, c->stream, a, (const uint16_t *)c->d_lbf);
+      ++k;
     }
   HIPCHK(c, hipGetLastError());
+  if (nlaunch) *nlaunch = k;
   return ESP_OK;
 }
 

@@ -225,6 +225,21 @@ def ferro_spins(N: int, R: int, mag=None) -> np.ndarray:
     return s
 
 
+def sublattice_spins(m, dirs, R: int = 1) -> np.ndarray:
+    """Spins [R][N][3] with basis atom a of every cell along dirs[a] (site index
+    cell * nb + a, jij.F90:219-229), scaled by the moments."""
+    d = np.asarray(dirs, float)
+    d = d / np.linalg.norm(d, axis=1, keepdims=True)
+    s = d[np.arange(m.N) % m.nb] * np.asarray(m.mag)[:, None]
+    return np.ascontiguousarray(np.repeat(s[None], R, axis=0))
+
+
+# The four-sublattice tetrahedral order of the FCC antiferromagnet: the four basis atoms of
+# the conventional cell point to the corners of a tetrahedron (all nearest-neighbour angles
+# arccos(-1/3), next-nearest neighbours parallel).
+FCC_TETRAHEDRAL = ((1, 1, 1), (1, -1, -1), (-1, 1, -1), (-1, -1, 1))
+
+
 # ---- the benchmark model configurations ------------------------------------------------
 def config_c1(L=8):
     """Classical Heisenberg ferromagnet, SC, J1 = 1; 16 temperatures 0.5-2.0 (lin)."""
