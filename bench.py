@@ -311,6 +311,34 @@ def run_learned(args):
     eng.close()
 
 
+# ---------------------------------------------------------------- counter calibration ------
+def run_copy_probe(args):
+    """Plain device copy of a known size (k_copy): the reference point that scales the
+    FETCH_SIZE / WRITE_SIZE counters into HBM bytes."""
+    from espins_amd.engine import Engine
+    n = 1 << 28
+    eng = Engine(0)
+    ms = eng.copy_probe(n, reps=max(1, args.steps))
+    gbs = 2 * n / (ms * 1e-3) / 1e9
+    log(f"k_copy: {n} B read + {n} B written per launch, {ms * 1e3:.1f} us -> {gbs:.0f} GB/s")
+    print(json.dumps({"metric": "device copy bandwidth (read + write)", "value": gbs, "unit": "GB/s",
+                      "n_gpus": 1, "steps": args.steps, "bytes_read_per_launch": n,
+                      "bytes_written_per_launch": n, "ms_per_launch": ms}), flush=True)
+    eng.close()
+
+
+def measured_traffic(config, kernel):
+    """HBM bytes per launch of `kernel` on `config`, from the committed rocprofv3 --pmc summary
+    profiles/pmc_<kernel>_<config>.txt (line 'traffic_bytes_per_launch = X'), or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{config}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.strip().startswith("traffic_bytes_per_launch"):
+            return float(line.split("=")[1])
+    return None
+
+
 # ---------------------------------------------------------------- main ---------------------
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
@@ -342,6 +370,8 @@ def main():
         return run_batch(args)
     if args.config == "learned_fcc32":
         return run_learned(args)
+    if args.config == "copy_probe":
+        return run_copy_probe(args)
     m, tems, desc, seed = workload(args.config)
     cpu = None
     if not args.no_cpu:   # before any GPU call (forked workers must not inherit a GPU context)
@@ -420,7 +450,7 @@ def main():
                    "exchange_every": 1 if pt else 0, "mode": "random", "colours": info["ncol"],
                    "kernel_path": "lds" if info["lds"] else "global", "graph": graph},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.config, kernel),
                      "kernel": kernel, "bytes_per_update": B,
                      "updates_per_launch": tm["updates_per_launch"],
                      "mean_launch_us": ms_launch * 1e3},

@@ -77,6 +77,7 @@ _SIGS = {
     "esp_get_trace": (_i, [_p, _dp, _ip]),
     "esp_sync": (_i, [_p]),
     "esp_stream": (_p, [_p]),
+    "esp_copy_probe": (_i, [_p, C.c_int64, _i, _dp]),
 }
 
 _fp = C.POINTER(C.c_float)
@@ -90,7 +91,7 @@ _SIGS.update({
     "esp_learn_put_records": (_i, [_p, _fp, _i]),
     "esp_learn_grad": (_i, [_p, _i, _i, _fp, _dp]),
     "esp_learn_train": (_i, [_p, _i, _i, C.c_float, _u64, _dp]),
-    "esp_learn_eval": (_i, [_p, _i, _i, _dp]),
+    "esp_learn_eval": (_i, [_p, _i, _i, _i, _dp]),
 })
 
 _SIGS.update({

@@ -29,6 +29,16 @@ constexpr int kRedBlocks = 64;    // reduce blocks per replica (upper bound)
 
 namespace {
 
+// Lanes per site: the caller's choice, else one lane per site on the global-memory path
+// (measured fastest there for z = 6..42: more sites in flight beat shorter bond loops) and,
+// in LDS, more lanes for wide bond lists, where a block has few sites per colour.
+int auto_tps(const esp_ctx *c) {
+  if (c->tps_user) return c->tps_user;
+  if (!c->use_lds) return 1;
+  const int z = c->hm.zmax;
+  return z <= 8 ? 1 : (z <= 20 ? 2 : 4);
+}
+
 // kernel tables: [mode][cmp][hb][hd][self]
 using PhaseFn = void (*)(KArgs);
 using LdsFn = void (*)(KArgs, int, LdsMeas);
@@ -340,9 +350,9 @@ int esp_set_model(esp_ctx *c, int N, int nb, const double *mag, int zJ, const in
   c->ctab_bytes = m.compressed ? ctab_bytes : 0;
   c->lds_bytes = (size_t)(N + 1) * sizeof(double4) + c->ctab_bytes + kLdsRed * sizeof(double);
   c->Ns = ((N + 1 + 63) / 64) * 64;
-  c->tps = c->tps_user ? c->tps_user : (m.zmax <= 8 ? 1 : (m.zmax <= 20 ? 2 : 4));
   c->lds_fits = c->lds_bytes + 4096 <= c->lds_max;
   c->use_lds = c->lds_fits && c->path != 1;
+  c->tps = auto_tps(c);
   if (c->lds_fits && c->lds_bytes > 65536) {
     // opt in to the large LDS allocation for every LDS kernel instance
     for (int mode = 0; mode < 2; ++mode) {
@@ -384,6 +394,7 @@ int esp_set_path(esp_ctx *c, int path) {
   if (c->have_model) {
     if (path == 2 && !c->lds_fits) return fail(c, ESP_EINVAL, "replica does not fit in LDS");
     c->use_lds = c->lds_fits && path != 1;
+    c->tps = auto_tps(c);
     drop_graph(c);
   }
   return ESP_OK;
@@ -393,8 +404,7 @@ int esp_set_tps(esp_ctx *c, int tps) {
   if (!c || !(tps == 0 || tps == 1 || tps == 2 || tps == 4 || tps == 8)) return ESP_EINVAL;
   c->tps_user = tps;
   if (c->have_model) {
-    const int z = c->hm.zmax;
-    c->tps = tps ? tps : (z <= 8 ? 1 : (z <= 20 ? 2 : 4));
+    c->tps = auto_tps(c);
     drop_graph(c);
   }
   return ESP_OK;
@@ -782,5 +792,43 @@ int esp_sync(esp_ctx *c) {
 }
 
 void *esp_stream(esp_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int esp_copy_probe(esp_ctx *c, int64_t n_bytes, int reps, double *ms) {
+  if (!c) return ESP_EINVAL;
+  if (n_bytes < 32 || n_bytes % 32 != 0 || n_bytes > ((int64_t)1 << 34) || reps < 1 || !ms)
+    return fail(c, ESP_EINVAL, "esp_copy_probe: n_bytes must be a positive multiple of 32 (<= 16 GiB)");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, ESP_EHIP, "hipSetDevice failed");
+  const size_t n = (size_t)n_bytes / sizeof(double4);
+  double4 *src = nullptr, *dst = nullptr;
+  int rc;
+  if ((rc = dalloc(c, src, n))) return rc;
+  if ((rc = dalloc(c, dst, n))) {
+    dfree(src);
+    return rc;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 1u << 20);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, c->stream, (const double4 *)src, dst, n);  // first touch
+    e = hipEventRecord(e0, c->stream);
+  }
+  for (int k = 0; e == hipSuccess && k < reps; ++k) {
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, c->stream, (const double4 *)src, dst, n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  dfree(src);
+  dfree(dst);
+  if (e != hipSuccess) return fail(c, ESP_EHIP, std::string("esp_copy_probe: ") + hipGetErrorString(e));
+  *ms = t / reps;
+  return ESP_OK;
+}
 
 }  // extern "C"

@@ -588,6 +588,13 @@ __global__ void k_get_spins(const double *sp, const int *ref2dev, const int *src
   p[0] = x[d]; p[1] = x[Ns + d]; p[2] = x[2 * Ns + d];
 }
 
+// Plain streaming copy (grid-stride, 32 B per lane per iteration): calibrates the HBM
+// traffic counters against a kernel whose bytes are known.
+__global__ void k_copy(const double4 *src, double4 *dst, size_t n) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+    dst[t] = src[t];
+}
+
 __global__ void k_fill(double *p, size_t n, double v) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = v;

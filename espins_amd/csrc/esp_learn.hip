@@ -43,6 +43,17 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr double kUniform = 1.0 / 32.0;
 
 __device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+// GELU with erf from Abramowitz & Stegun 7.1.26 (|error| <= 1.5e-7): branch-free, one exp2
+// and one reciprocal.  The activation of the sweep's network.
+__device__ __forceinline__ float gelu_fast(float x) {
+  const float y = fabsf(x) * 0.70710678118654752f;
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, y, 1.0f));
+  const float p =
+      t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 1.061405429f, -1.453152027f), 1.421413741f), -0.284496736f), 0.254829592f);
+  const float e = p * __builtin_amdgcn_exp2f(-y * y * 1.4426950408889634f);   // 1 - erf(|x| / sqrt 2)
+  return 0.5f * x * (x >= 0.0f ? 2.0f - e : e);
+}
+
 __device__ __forceinline__ float gelu_d(float x) {
   return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
 }
@@ -80,14 +91,169 @@ __device__ __forceinline__ void mlp_forward(const float *w, const float x[kIn], 
   }
 }
 
+// log(1 - kUniform), log(kUniform / (4 pi)), log(2 pi)
+constexpr double kLog1mU = -0.0317486983145803;
+constexpr double kLogUni = -5.996760149769017;
+constexpr double kLog2Pi = 1.8378770664093453;
+
+// The proposal density of one site:
+//   q(x) = (1 - kUniform) sum_k w_k vMF(x; kappa_k, mu_k) + kUniform / (4 pi).
+// The parameters (w, kappa, mu) are computed from the network outputs in FP32 -- any
+// values work, provided sampling and density use the same ones -- and promoted to FP64;
+// the density, its normaliser and the sampler are FP64.
+// ---- the sweep's network: hidden layers on the matrix cores --------------------------------
+// One wave evaluates the network for its 64 lanes' inputs.  Layer 1 (5 -> 32) and the output
+// layer (32 -> 6) run per lane in FP32; the two 32 x 32 layers run as
+//   Z^T = W H^T   (v_mfma_f32_32x32x16_bf16: A = W, B = H^T, D = Z^T)
+// with BF16 weights and activations and FP32 accumulation, two 32-proposal column blocks per
+// wave.  Operand layouts of the instruction (lane l, half h = l / 32):
+//   A[32][16]: A[l % 32][8 h + e], e < 8        B[16][32]: B[8 h + e][l % 32]
+//   D[32][32]: register v holds D[8 (v / 4) + 4 h + v % 4][l % 32]
+// A lane's D registers cover hidden units {8 q + 4 h + r} of one proposal; the B fragment of
+// the next layer needs units {16 ks + 8 h + e}: half of them are the partner lane's (l ^ 32),
+// exchanged with one cross-half shuffle per register pair.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {   // round to nearest even
+  const uint32_t a = __float_as_uint(lo), b = __float_as_uint(hi);
+  const uint32_t ra = (a + 0x7FFFu + ((a >> 16) & 1u)) >> 16;
+  const uint32_t rb = (b + 0x7FFFu + ((b >> 16) & 1u)) >> 16;
+  return ra | (rb << 16);
+}
+
+__device__ __forceinline__ bf16x8 frag(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3) {
+  uint4 u = make_uint4(d0, d1, d2, d3);
+  return __builtin_bit_cast(bf16x8, u);
+}
+
+// W2 / W3 A-fragments (BF16, from global memory): row l % 32, columns 16 ks + 8 h + 0..7.
+struct NetA {
+  bf16x8 a[2][2];   // [layer 2 / 3][ks]
+};
+__device__ __forceinline__ NetA load_net_a(const uint16_t *wbf16) {
+  NetA n;
+  const int l = threadIdx.x & 63, h = l >> 5, i = l & 31;
+#pragma unroll
+  for (int layer = 0; layer < 2; ++layer)
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      const uint4 u = *reinterpret_cast<const uint4 *>(wbf16 + (layer ? oW3 : oW2) + i * kH + 16 * ks + 8 * h);
+      n.a[layer][ks] = __builtin_bit_cast(bf16x8, u);
+    }
+  return n;
+}
+
+// B fragments of one 32-proposal column block from D registers (after the activation):
+// keep q in {h, 2 + h}, exchange q in {1 - h, 3 - h} with the partner lane.
+__device__ __forceinline__ void d_to_b(const floatx16 &z, int h, bf16x8 b[2]) {
+  uint32_t p[8];   // p[2 q + r / 2]: units 8 q + 4 h + r, r = 0..3, as BF16 pairs
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    p[2 * q] = pack_bf16(z[4 * q], z[4 * q + 1]);
+    p[2 * q + 1] = pack_bf16(z[4 * q + 2], z[4 * q + 3]);
+  }
+  // send q = 1 - h and q = 3 - h; receive the partner's q = h' for the same slots
+  const uint32_t s0 = h ? p[0] : p[2], s1 = h ? p[1] : p[3];
+  const uint32_t s2 = h ? p[4] : p[6], s3 = h ? p[5] : p[7];
+  const uint32_t r0 = __shfl_xor(s0, 32, 64), r1 = __shfl_xor(s1, 32, 64);
+  const uint32_t r2 = __shfl_xor(s2, 32, 64), r3 = __shfl_xor(s3, 32, 64);
+  // ks = 0: units 8 h + 0..7; ks = 1: units 16 + 8 h + 0..7
+  //   h = 0: [own q = 2 ks | partner q = 2 ks];   h = 1: [partner q = 2 ks + 1 | own q = 2 ks + 1]
+  b[0] = h ? frag(r0, r1, p[2], p[3]) : frag(p[0], p[1], r0, r1);
+  b[1] = h ? frag(r2, r3, p[6], p[7]) : frag(p[4], p[5], r2, r3);
+}
+
+__device__ __forceinline__ floatx16 bias_d(const float *b, int h) {
+  floatx16 z;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) z[v] = b[8 * (v >> 2) + 4 * h + (v & 3)];
+  return z;
+}
+
+// Network of the sweep for the calling lane's input x (every lane of the wave must call).
+// wl: FP32 weights in LDS (layer 1, biases, layer 4 from the BF16 copy); na: W2/W3 fragments.
+__device__ __forceinline__ void net_forward_mfma(const float *wl, const NetA &na, const float x[kIn],
+                                                 float out[kOut]) {
+  const int l = threadIdx.x & 63, h = l >> 5;
+  // layer 1 per lane, then B fragments: own units 16 ks + 8 h + e for column block h, the
+  // partner's same units for column block 1 - h
+  uint32_t own[8], snd[8];
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+    for (int half = 0; half < 2; ++half)
+#pragma unroll
+      for (int e = 0; e < 8; e += 2) {
+        float z[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          const int u = 16 * ks + 8 * (half ^ h) + e + t;   // half 0: own units, 1: the partner's
+          float acc = wl[oB1 + u];
+#pragma unroll
+          for (int k = 0; k < kIn; ++k) acc += wl[oW1 + u * kIn + k] * x[k];
+          z[t] = gelu_fast(acc);
+        }
+        (half ? snd : own)[4 * ks + e / 2] = pack_bf16(z[0], z[1]);
+      }
+  uint32_t rcv[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) rcv[k] = __shfl_xor(snd[k], 32, 64);
+  bf16x8 b[2][2];   // [column block][ks]
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks) {
+    const bf16x8 mine = frag(own[4 * ks], own[4 * ks + 1], own[4 * ks + 2], own[4 * ks + 3]);
+    const bf16x8 theirs = frag(rcv[4 * ks], rcv[4 * ks + 1], rcv[4 * ks + 2], rcv[4 * ks + 3]);
+    b[0][ks] = h ? theirs : mine;
+    b[1][ks] = h ? mine : theirs;
+  }
+  // layers 2 and 3 on the matrix cores
+  floatx16 z[2];
+#pragma unroll
+  for (int layer = 0; layer < 2; ++layer) {
+    const float *bias = wl + (layer ? oB3 : oB2);
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      z[cb] = bias_d(bias, h);
+      z[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(na.a[layer][0], b[cb][0], z[cb], 0, 0, 0);
+      z[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(na.a[layer][1], b[cb][1], z[cb], 0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < 16; ++v) z[cb][v] = gelu_fast(z[cb][v]);
+    }
+    if (layer == 0)
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) d_to_b(z[cb], h, b[cb]);
+  }
+  // output layer: partial sums over this lane's 16 units of each column block, completed
+  // with the partner's; the lane's own proposal is column block h
+#pragma unroll
+  for (int q = 0; q < kOut; ++q) {
+    float part[2];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      float acc = 0.0f;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc += wl[oW4 + q * kH + 8 * (v >> 2) + 4 * h + (v & 3)] * z[cb][v];
+      part[cb] = acc;
+    }
+    const float give = h ? part[0] : part[1];     // the partner's column block
+    const float got = __shfl_xor(give, 32, 64);
+    out[q] = wl[oB4 + q] + (h ? part[1] : part[0]) + got;
+  }
+}
+
 struct Mix {
-  double kappa[2], logw[2];
+  double logw[2];   // log of (1 - kUniform) w_k
+  double kappa[2];
+  double lnorm[2];  // log of kappa / (2 pi (1 - e^{-2 kappa})): vMF density = exp(lnorm + kappa (mu.x - 1))
+  double e2k[2];    // e^{-2 kappa}
+  double t[2];      // tilt angle of mu_k from ghat
   V3 mu[2];
 };
 
 __device__ __forceinline__ V3 vnorm(V3 v) {
-  const double n = sqrt(dot(v, v));
-  return {v.x / n, v.y / n, v.z / n};
+  const double r = rsqrt(dot(v, v));
+  return {v.x * r, v.y * r, v.z * r};
 }
 
 // Unit vector in the (ghat, axis) plane, normal to ghat, toward the pole of the axis nearer
@@ -96,58 +262,55 @@ __device__ __forceinline__ V3 tilt_frame(const V3 &ghat, const V3 &ax) {
   const double c = dot(ghat, ax);
   const double sgn = c < 0.0 ? -1.0 : 1.0;
   V3 e = {sgn * ax.x - sgn * c * ghat.x, sgn * ax.y - sgn * c * ghat.y, sgn * ax.z - sgn * c * ghat.z};
-  double en = sqrt(dot(e, e));
-  if (en < 1e-12) {
-    e = fabs(ghat.x) < 0.9 ? V3{0.0, -ghat.z, ghat.y} : V3{-ghat.z, 0.0, ghat.x};
-    en = sqrt(dot(e, e));
-  }
-  return {e.x / en, e.y / en, e.z / en};
+  if (dot(e, e) < 1e-24) e = fabs(ghat.x) < 0.9 ? V3{0.0, -ghat.z, ghat.y} : V3{-ghat.z, 0.0, ghat.x};
+  return vnorm(e);
 }
 
+__device__ __forceinline__ float softplusf(float z) { return fmaxf(z, 0.0f) + log1pf(expf(-fabsf(z))); }
+
 // Mixture from the network outputs and the local frame (ghat, pole axis).
-__device__ __forceinline__ Mix make_mix(const float o[kOut], double kappa0, const V3 &ghat, const V3 &ax) {
+__device__ __forceinline__ Mix make_mix(const float o[kOut], float kappa0, const V3 &ghat, const V3 &ax) {
   Mix m;
   const V3 e = tilt_frame(ghat, ax);
-  const double a0 = o[1], a1 = o[4];
-  const double amax = fmax(a0, a1);
-  const double lse = amax + log(exp(a0 - amax) + exp(a1 - amax));
-  m.logw[0] = a0 - lse + log1p(-kUniform);   // (1 - kUniform) x softmax weight
-  m.logw[1] = a1 - lse + log1p(-kUniform);
+  const float d = o[4] - o[1];                 // log softmax of the two weight logits
+  m.logw[0] = (double)(-softplusf(d)) + kLog1mU;
+  m.logw[1] = (double)(-softplusf(-d)) + kLog1mU;
+#pragma unroll
   for (int k = 0; k < 2; ++k) {
-    const double s = fmin(fmax((double)o[3 * k], -8.0), 8.0);
-    m.kappa[k] = kappa0 * exp(s) + 1e-6;
-    const double t = 0.5 * kPi * tanh((double)o[3 * k + 2]);
-    const double ct = cos(t), st = sin(t);
-    m.mu[k] = {ct * ghat.x + st * e.x, ct * ghat.y + st * e.y, ct * ghat.z + st * e.z};
+    const float sc = fminf(fmaxf(o[3 * k], -8.0f), 8.0f);
+    const double kap = (double)(kappa0 * expf(sc)) + 1e-6;
+    m.kappa[k] = kap;
+    const double em1 = expm1(-2.0 * kap);
+    m.e2k[k] = 1.0 + em1;
+    m.lnorm[k] = log(kap / -em1) - kLog2Pi;
+    const float th = tanhf(o[3 * k + 2]);
+    float sn, cs;
+    sincospif(0.5f * th, &sn, &cs);
+    m.t[k] = 0.5 * kPi * (double)th;
+    m.mu[k] = vnorm(V3{cs * ghat.x + sn * e.x, cs * ghat.y + sn * e.y, cs * ghat.z + sn * e.z});
   }
   return m;
 }
 
-// log of the vMF density on the unit sphere: kappa/(4 pi sinh kappa) exp(kappa mu.x)
-__device__ __forceinline__ double log_vmf(double kappa, const V3 &mu, const V3 &x) {
-  if (kappa < 1e-8) return -log(4.0 * kPi);
-  return log(kappa) - log(2.0 * kPi) - log1p(-exp(-2.0 * kappa)) + kappa * (dot(mu, x) - 1.0);
+// log vMF_k(x) = lnorm_k + kappa_k (mu_k . x - 1)
+__device__ __forceinline__ double log_vmf(const Mix &m, int k, const V3 &x) {
+  return m.lnorm[k] + m.kappa[k] * (dot(m.mu[k], x) - 1.0);
 }
-
-__device__ __forceinline__ double log_uniform() { return log(kUniform / (4.0 * kPi)); }
 
 __device__ __forceinline__ double log_q(const Mix &m, const V3 &x) {
-  const double l0 = m.logw[0] + log_vmf(m.kappa[0], m.mu[0], x);
-  const double l1 = m.logw[1] + log_vmf(m.kappa[1], m.mu[1], x);
-  const double lu = log_uniform();
-  const double mx = fmax(fmax(l0, l1), lu);
-  return mx + log(exp(l0 - mx) + exp(l1 - mx) + exp(lu - mx));
+  const double l0 = m.logw[0] + log_vmf(m, 0, x);
+  const double l1 = m.logw[1] + log_vmf(m, 1, x);
+  const double mx = fmax(fmax(l0, l1), kLogUni);
+  return mx + log(exp(l0 - mx) + exp(l1 - mx) + exp(kLogUni - mx));
 }
 
-// Wood (1994) for S^2: w = 1 + log(v + (1-v) e^{-2 kappa}) / kappa, azimuth 2 pi u.
-__device__ __forceinline__ V3 sample_vmf(double kappa, const V3 &mu, double v, double u) {
-  double w;
-  if (kappa < 1e-8) w = 2.0 * v - 1.0;
-  else w = 1.0 + log(v + (1.0 - v) * exp(-2.0 * kappa)) / kappa;
+// Wood (1994) for S^2: w = 1 + log(v + (1 - v) e^{-2 kappa}) / kappa, azimuth 2 pi u; kappa = 0
+// is the uniform density (w = 2 v - 1).
+__device__ __forceinline__ V3 sample_vmf(double kappa, double e2k, const V3 &mu, double v, double u) {
+  double w = kappa > 0.0 ? 1.0 + log(v + (1.0 - v) * e2k) / kappa : 2.0 * v - 1.0;
   w = fmin(1.0, fmax(-1.0, w));
   const double st = sqrt(fmax(0.0, 1.0 - w * w));
-  V3 e1 = fabs(mu.x) < 0.9 ? V3{0.0, -mu.z, mu.y} : V3{-mu.z, 0.0, mu.x};
-  e1 = vnorm(e1);
+  const V3 e1 = vnorm(fabs(mu.x) < 0.9 ? V3{0.0, -mu.z, mu.y} : V3{-mu.z, 0.0, mu.x});
   const V3 e2 = {mu.y * e1.z - mu.z * e1.y, mu.z * e1.x - mu.x * e1.z, mu.x * e1.y - mu.y * e1.x};
   double sp, cp;
   sincospi(2.0 * u, &sp, &cp);
@@ -222,18 +385,24 @@ __device__ __forceinline__ double quad(const double Q[6], const V3 &s) {
 //   x3 = beta mag^2 (ghat^T Q_B ghat - tr Q_B / 3)   biquadratic projection on the field
 //   x4 = beta mag^2 Delta                   single-ion anisotropy
 __device__ __forceinline__ void features(const V3 &G, const double QB[6], const V3 &ax, double Delta,
-                                         double beta, double mag, float x[kIn], double &kappa0,
-                                         V3 &ghat) {
-  const double g = sqrt(dot(G, G));
-  ghat = g > 1e-300 ? V3{G.x / g, G.y / g, G.z / g} : ax;
-  kappa0 = beta * mag * g;
-  const double bm2 = beta * mag * mag;
-  const double trq = QB[0] + QB[1] + QB[2];
-  x[0] = (float)log1p(kappa0);
-  x[1] = (float)fabs(dot(ghat, ax));
-  x[2] = (float)log(1.0 / beta);
-  x[3] = (float)(bm2 * (quad(QB, ghat) - trq / 3.0));
-  x[4] = (float)(bm2 * Delta);
+                                         double beta, double mag, float x[kIn], float &kappa0, V3 &ghat) {
+  const double g2 = dot(G, G);
+  double g = 0.0;
+  if (g2 > 1e-300) {
+    g = sqrt(g2);
+    const double r = 1.0 / g;
+    ghat = {G.x * r, G.y * r, G.z * r};
+  } else {
+    ghat = ax;
+  }
+  kappa0 = (float)(beta * mag * g);
+  const float bm2 = (float)(beta * mag * mag);
+  const float trq = (float)(QB[0] + QB[1] + QB[2]);
+  x[0] = log1pf(kappa0);
+  x[1] = fabsf((float)dot(ghat, ax));
+  x[2] = logf((float)(1.0 / beta));
+  x[3] = bm2 * ((float)quad(QB, ghat) - trq * (1.0f / 3.0f));
+  x[4] = bm2 * (float)Delta;
 }
 
 __device__ __forceinline__ void axis_of(const KArgs &a, int d, V3 &ax, double &Delta) {
@@ -260,9 +429,11 @@ __global__ __launch_bounds__(kBlock) void k_phase_learned(KArgs a, const uint16_
   for (int k = threadIdx.x; k < nct; k += kBlock) ctab[k] = a.ctab[k];
   for (int k = threadIdx.x; k < kNP; k += kBlock) w[k] = __uint_as_float((uint32_t)wbf16[k] << 16);
   __syncthreads();
+  // every lane runs the network (matrix instructions need the whole wave); lanes past the
+  // colour class evaluate a stand-in site and write nothing
   const int k = b * kBlock + threadIdx.x;
-  if (k >= a.c_count) return;  // no barriers below
-  const int d = a.c_begin + k;
+  const bool active = k < a.c_count;
+  const int d = a.c_begin + (active ? k : 0);
   const int slot = a.rep_slot[r];
   const double beta = a.beta_slot[slot];
   const unsigned long long sweep = a.clock->sweep + (unsigned long long)a.sweep_off;
@@ -278,10 +449,10 @@ __global__ __launch_bounds__(kBlock) void k_phase_learned(KArgs a, const uint16_
                  QB[3] + Delta * ax.x * ax.y, QB[4] + Delta * ax.x * ax.z, QB[5] + Delta * ax.y * ax.z};
   const double mag = a.mag[d];
   float x[kIn], o[kOut];
-  double kappa0;
+  float kappa0;
   V3 ghat;
   features(G, QB, ax, Delta, beta, mag, x, kappa0, ghat);
-  mlp_forward(w, x, o);
+  net_forward_mfma(w, load_net_a(wbf16), x, o);
   const Mix mix = make_mix(o, kappa0, ghat, ax);
   const int ir = a.dev2ref[d];
   const uint4 rn = philox10(make_uint4((uint32_t)ir, (uint32_t)sweep, (uint32_t)(sweep >> 32),
@@ -290,19 +461,30 @@ __global__ __launch_bounds__(kBlock) void k_phase_learned(KArgs a, const uint16_
   // component: uniform with probability kUniform, else network component 0 or 1
   const double uc = u01(rn.x);
   const int comp = uc < kUniform ? 2 : (uc - kUniform < exp(mix.logw[0]) ? 0 : 1);
-  const V3 xn = comp == 2 ? sample_vmf(0.0, ghat, u01(rn.y), u01(rn.z))
-                          : sample_vmf(mix.kappa[comp], mix.mu[comp], u01(rn.y), u01(rn.z));
+  // select by value (a dynamic index into the register-resident mixture would spill it)
+  const bool one = comp == 1;
+  const V3 xn = sample_vmf(comp == 2 ? 0.0 : (one ? mix.kappa[1] : mix.kappa[0]),
+                           comp == 2 ? 1.0 : (one ? mix.e2k[1] : mix.e2k[0]),
+                           comp == 2 ? ghat : (one ? mix.mu[1] : mix.mu[0]), u01(rn.y), u01(rn.z));
   const V3 so = sp.ld((unsigned)d);
   const V3 sn = {mag * xn.x, mag * xn.y, mag * xn.z};
   const V3 xo = vnorm(so);
   const double dE = -(G.x * (sn.x - so.x) + G.y * (sn.y - so.y) + G.z * (sn.z - so.z)) + quad(Q, sn) - quad(Q, so);
-  const double logr = -beta * dE + log_q(mix, xo) - log_q(mix, xn);
-  const bool acc = logr >= 0.0 || u01(rn.w) < exp(logr);
-  if (acc) {
+  // Metropolis-Hastings ratio exp(-beta dE) q(xo) / q(xn) with q in the linear domain: the
+  // uniform component keeps both densities >= kUniform / (4 pi) > 0
+  const double c0 = exp(mix.logw[0] + mix.lnorm[0]), c1 = exp(mix.logw[1] + mix.lnorm[1]);
+  const double cu = kUniform / (4.0 * kPi);
+  const double qo = c0 * exp(mix.kappa[0] * (dot(mix.mu[0], xo) - 1.0)) +
+                    c1 * exp(mix.kappa[1] * (dot(mix.mu[1], xo) - 1.0)) + cu;
+  const double qn = c0 * exp(mix.kappa[0] * (dot(mix.mu[0], xn) - 1.0)) +
+                    c1 * exp(mix.kappa[1] * (dot(mix.mu[1], xn) - 1.0)) + cu;
+  const double ratio = exp(-beta * dE) * qo / qn;
+  const bool acc = ratio >= 1.0 || u01(rn.w) < ratio;
+  if (active && acc) {
     sp.x[d] = sn.x; sp.y[d] = sn.y; sp.z[d] = sn.z;
     a.w[(size_t)r * a.Ns + d] += dE;
   }
-  const unsigned long long m = __ballot(!acc);
+  const unsigned long long m = __ballot(active && !acc);
   if ((threadIdx.x & 63) == 0 && m) atomicAdd(&a.rej[slot], (unsigned long long)__popcll(m));
 }
 
@@ -328,13 +510,13 @@ __global__ __launch_bounds__(kBlock) void k_collect(KArgs a, int n, uint32_t s0,
   axis_of(a, d, ax, Delta);
   const double beta = a.beta_slot[a.rep_slot[r]];
   float x[kIn];
-  double kappa0;
+  float kappa0;
   V3 ghat;
   features(G, QB, ax, Delta, beta, a.mag[d], x, kappa0, ghat);
   const V3 s = vnorm(sp.ld((unsigned)d));
   float *rec = out + (size_t)t * kRec;
   for (int k = 0; k < kIn; ++k) rec[k] = x[k];
-  rec[5] = (float)kappa0;
+  rec[5] = kappa0;
   rec[6] = (float)ghat.x; rec[7] = (float)ghat.y; rec[8] = (float)ghat.z;
   rec[9] = (float)ax.x; rec[10] = (float)ax.y; rec[11] = (float)ax.z;
   rec[12] = (float)s.x; rec[13] = (float)s.y; rec[14] = (float)s.z;
@@ -389,14 +571,12 @@ __global__ __launch_bounds__(kTrainBlock) void k_train_block(const float *w, con
       for (int k = 0; k < kH; ++k) z += w[oW4 + q * kH + k] * a3[k];
       o[q] = z;
     }
-    const double kappa0 = rec[5];
     const V3 ghat = rec_vec(rec, 6), ax = rec_vec(rec, 9), s = rec_vec(rec, 12);
-    const Mix m = make_mix(o, kappa0, ghat, ax);
+    const Mix m = make_mix(o, rec[5], ghat, ax);
     double lv[2];
-    for (int k = 0; k < 2; ++k) lv[k] = m.logw[k] + log_vmf(m.kappa[k], m.mu[k], s);
-    const double lu = log_uniform();
-    const double mx = fmax(fmax(lv[0], lv[1]), lu);
-    const double lse = mx + log(exp(lv[0] - mx) + exp(lv[1] - mx) + exp(lu - mx));
+    for (int k = 0; k < 2; ++k) lv[k] = m.logw[k] + log_vmf(m, k, s);
+    const double mx = fmax(fmax(lv[0], lv[1]), kLogUni);
+    const double lse = mx + log(exp(lv[0] - mx) + exp(lv[1] - mx) + exp(kLogUni - mx));
     loss = -lse;
     const double rnet = exp(lv[0] - lse) + exp(lv[1] - lse);  // responsibility of the network part
     // d loss / d outputs
@@ -408,9 +588,9 @@ __global__ __launch_bounds__(kTrainBlock) void k_train_block(const float *w, con
       const double sraw = (double)o[3 * k];
       const double dks = (sraw > -8.0 && sraw < 8.0) ? (kap - 1e-6) : 0.0;              // d kappa / d o
       g4[3 * k] = (float)(-rk * dlk * dks);
-      g4[3 * k + 1] = (float)(exp(m.logw[k]) / (1.0 - kUniform) * rnet - rk);
-      const double th = tanh((double)o[3 * k + 2]);
-      const double t = 0.5 * kPi * th;
+      g4[3 * k + 1] = (float)(exp(m.logw[k] - kLog1mU) * rnet - rk);
+      const double t = m.t[k];
+      const double th = t / (0.5 * kPi);
       const V3 dmu = {-sin(t) * ghat.x + cos(t) * e.x, -sin(t) * ghat.y + cos(t) * e.y,
                       -sin(t) * ghat.z + cos(t) * e.z};
       g4[3 * k + 2] = (float)(-rk * kap * dot(dmu, s) * 0.5 * kPi * (1.0 - th * th));
@@ -506,14 +686,27 @@ __global__ void k_to_bf16(const float *w, uint16_t *wbf16) {
   wbf16[p] = (uint16_t)((u + rb) >> 16);
 }
 
-// Evaluation: per record the six outputs, kappa_0, kappa_1, log q(s | env) and w_0.
-__global__ void k_eval(const float *w, const float *recs, int n, double *out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float *rec = recs + (size_t)i * kRec;
+// Evaluation: per record the six outputs, kappa_0, kappa_1, log q(s | env) and w_0, from the
+// FP32 master network (sweep_net = 0) or from the network the sweep runs (BF16 weights and
+// hidden activations on the matrix cores, sweep_net = 1).
+constexpr int kEvalBlock = 128;
+__global__ __launch_bounds__(kEvalBlock) void k_eval(const float *w, const uint16_t *wbf16, int sweep_net,
+                                                     const float *recs, int n, double *out) {
+  __shared__ float wl[kNP];
+  const int i = blockIdx.x * kEvalBlock + threadIdx.x;
+  const int ic = i < n ? i : n - 1;            // whole waves run the matrix-core network
+  const float *rec = recs + (size_t)ic * kRec;
   float x[kIn], o[kOut];
   for (int k = 0; k < kIn; ++k) x[k] = rec[k];
-  mlp_forward(w, x, o);
+  if (sweep_net) {
+    for (int k = threadIdx.x; k < kNP; k += kEvalBlock) wl[k] = __uint_as_float((uint32_t)wbf16[k] << 16);
+    const NetA na = load_net_a(wbf16);
+    __syncthreads();
+    net_forward_mfma(wl, na, x, o);
+  } else {
+    mlp_forward(w, x, o);
+  }
+  if (i >= n) return;
   const V3 ghat = rec_vec(rec, 6), ax = rec_vec(rec, 9), s = rec_vec(rec, 12);
   const Mix m = make_mix(o, rec[5], ghat, ax);
   double *q = out + (size_t)i * 10;
@@ -521,7 +714,7 @@ __global__ void k_eval(const float *w, const float *recs, int n, double *out) {
   q[6] = m.kappa[0];
   q[7] = m.kappa[1];
   q[8] = log_q(m, s);
-  q[9] = exp(m.logw[0]) / (1.0 - kUniform);
+  q[9] = exp(m.logw[0] - kLog1mU);
 }
 
 using LearnFn = void (*)(KArgs, const uint16_t *);
@@ -799,7 +992,7 @@ int esp_learn_train(esp_ctx *c, int epochs, int batch, float lr, uint64_t seed, 
   return ESP_OK;
 }
 
-int esp_learn_eval(esp_ctx *c, int first, int n, double *out) {
+int esp_learn_eval(esp_ctx *c, int first, int n, int sweep_net, double *out) {
   int rc = learn_ready(c, false);
   if (rc) return rc;
   if (!c->have_learn) return fail(c, ESP_ESTATE, "no proposal weights");
@@ -807,7 +1000,8 @@ int esp_learn_eval(esp_ctx *c, int first, int n, double *out) {
   if (n == 0) return ESP_OK;
   double *d_out = nullptr;
   if ((rc = dalloc(c, d_out, (size_t)n * 10))) return rc;
-  hipLaunchKernelGGL(k_eval, dim3((n + 127) / 128), dim3(128), 0, c->stream, (const float *)c->d_lw,
+  hipLaunchKernelGGL(k_eval, dim3((n + kEvalBlock - 1) / kEvalBlock), dim3(kEvalBlock), 0, c->stream,
+                     (const float *)c->d_lw, (const uint16_t *)c->d_lbf, sweep_net ? 1 : 0,
                      (const float *)c->d_recs + (size_t)kRec * first, n, d_out);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, sizeof(double) * 10 * n, hipMemcpyDeviceToHost, c->stream);

@@ -225,6 +225,12 @@ class Engine:
     def stream(self):
         return E.lib.esp_stream(self._h)
 
+    def copy_probe(self, n_bytes, reps=10):
+        """Mean ms per launch of a plain n_bytes device copy (counter calibration)."""
+        ms = C.c_double()
+        self._chk(E.lib.esp_copy_probe(self._h, int(n_bytes), int(reps), C.byref(ms)))
+        return ms.value
+
     # ---- learned proposal -------------------------------------------------------------
     def learn_set_weights(self, w):
         w = E.f32(w)
@@ -270,10 +276,11 @@ class Engine:
                                         E.ptr(out, C.c_double)))
         return out[:epochs]
 
-    def learn_eval(self, first=0, n=None):
+    def learn_eval(self, first=0, n=None, sweep_net=False):
         n = self.learn_num_records() - first if n is None else n
         out = np.zeros((n, 10))
-        self._chk(E.lib.esp_learn_eval(self._h, int(first), int(n), E.ptr(out, C.c_double)))
+        self._chk(E.lib.esp_learn_eval(self._h, int(first), int(n), int(bool(sweep_net)),
+                                       E.ptr(out, C.c_double)))
         return out
 
 

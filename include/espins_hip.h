@@ -165,6 +165,11 @@ int esp_get_trace(esp_ctx *ctx, double *trace, int *n);
 int esp_sync(esp_ctx *ctx);
 /* The context's HIP stream (hipStream_t) for callers that time or order work on it. */
 void *esp_stream(esp_ctx *ctx);
+/* Diagnostics, no counterpart in the reference: a plain device-to-device copy kernel
+ * (k_copy) of n_bytes, launched reps times after one untimed launch; *ms = mean time per
+ * launch.  Calibrates HBM traffic counters (FETCH_SIZE / WRITE_SIZE) and the copy
+ * bandwidth the sweep kernels are compared with. */
+int esp_copy_probe(esp_ctx *ctx, int64_t n_bytes, int reps, double *ms);
 
 /* ---- learned proposal (no counterpart in the reference) --------------------------------
  * mode ESP_MODE_LEARNED in esp_sweep / esp_run draws each new spin from a two-component
@@ -196,8 +201,10 @@ int esp_learn_grad(esp_ctx *ctx, int first, int n, float *grad, double *loss);
  * of `batch`; epoch_loss[epochs] (may be NULL) = mean loss seen in each epoch. */
 int esp_learn_train(esp_ctx *ctx, int epochs, int batch, float lr, uint64_t seed,
                     double *epoch_loss);
-/* Per record [n][10]: the six outputs, kappa_0, kappa_1, log q(s | env), w_0. */
-int esp_learn_eval(esp_ctx *ctx, int first, int n, double *out);
+/* Per record [n][10]: the six outputs, kappa_0, kappa_1, log q(s | env), w_0; from the FP32
+ * master weights (sweep_net = 0) or from the network the sweep runs (sweep_net = 1: BF16
+ * weights and hidden activations on the matrix cores, FP32 accumulation). */
+int esp_learn_eval(esp_ctx *ctx, int first, int n, int sweep_net, double *out);
 
 /* ---- batches of independent chains -------------------------------------------------- */
 /* Many PT chains (lattices of different kind and size, each with its own ladder) advanced

@@ -68,6 +68,23 @@ def forward(w, X):
     return o, (X, z1, a1, z2, a2, z3, a3)
 
 
+def bf16_round(a):
+    """Round to the nearest BF16 (ties to even), returned as float64."""
+    u = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def forward_sweep_net(w, X):
+    """The network as the sweep evaluates it: BF16 weights; the inputs of the two 32 x 32
+    layers rounded to BF16; sums and activations otherwise in full precision."""
+    p = unpack(bf16_round(w))
+    a1 = bf16_round(gelu(X @ p["W1"].T + p["b1"]))
+    a2 = bf16_round(gelu(a1 @ p["W2"].T + p["b2"]))
+    a3 = gelu(a2 @ p["W3"].T + p["b3"])
+    return a3 @ p["W4"].T + p["b4"]
+
+
 def _unit(v):
     v = np.asarray(v, np.float64)
     return v / np.linalg.norm(v, axis=-1, keepdims=True)

@@ -63,6 +63,31 @@ def test_eval_matches_oracle(Engine):
     assert np.all(np.abs(out[:, 8] - lq) <= 1e-4 * (np.abs(lq) + np.maximum(1.0, k0)))
 
 
+def test_sweep_network_matches_oracle(Engine):
+    """The network the sweep runs (BF16 weights and hidden activations, matrix cores) against
+    the oracle's restatement of that arithmetic, and within BF16 error of the FP32 network."""
+    recs = random_records(1000, 26).astype(np.float32)   # 1000: a partial last wave
+    w = weights32(27, out_scale=0.5)
+    with Engine() as eng:
+        eng.learn_set_weights(w)
+        eng.learn_put_records(recs)
+        out = eng.learn_eval(sweep_net=True)
+        ref32 = eng.learn_eval(sweep_net=False)
+    X = recs[:, :5].astype(np.float64)
+    o = LO.forward_sweep_net(w.astype(np.float64), X)
+    # agreement to FP32 rounding, except where an activation sits on a BF16 rounding
+    # boundary and the two roundings differ by one BF16 step
+    d = np.abs(out[:, :6] - o)
+    scale = np.abs(ref32[:, :6]).max()
+    assert np.mean(d > 1e-4 * (1 + np.abs(o))) < 2e-3, np.sort(d.ravel())[-10:]
+    assert d.max() < 0.01 * scale
+    assert np.abs(out[:, :6] - ref32[:, :6]).max() < 0.05 * scale
+    # the mixture built from the sweep network is a normalised density as well
+    X_, k0, gh, ax, s = LO.split_records(recs)
+    lq = LO.log_q(out[:, :6], k0, gh, ax, s)
+    assert np.all(np.abs(out[:, 8] - lq) <= 1e-4 * (np.abs(lq) + np.maximum(1.0, k0)))
+
+
 def test_one_adam_step(Engine):
     """First Adam step from zero moments: w - lr * g / (|g| + eps)."""
     recs = random_records(512, 24).astype(np.float32)
