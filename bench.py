@@ -432,6 +432,13 @@ def main():
     kernel = "k_lds" if info["lds"] else "k_phase"
     log(f"{kernel}: {tm['nlaunch']} launches, mean {ms_launch * 1e3:.2f} us, "
         f"{tm['updates_per_launch']} updates/launch, B_upd {B:.0f} B -> {achieved:.0f} GB/s")
+    # measured copy bandwidth of this device (read + write of 256 MiB), and the FP64 work of
+    # the sweep (70 + 6 zJ + 12 zB + 11 zD flops per update) against the FP64 vector peak
+    copy_gbs = 2 * (1 << 28) / (eng.copy_probe(1 << 28, reps=10) * 1e-3) / 1e9
+    zJ, zB, zD = m.J.z, (m.B.z if m.B else 0), (m.D.z if m.D else 0)
+    flops = 70 + 6 * zJ + 12 * zB + 11 * zD
+    log(f"copy bandwidth {copy_gbs:.0f} GB/s; sweep FP64 work {flops} flop/update -> "
+        f"{value * flops / 1e12:.2f} TFLOP/s of 78.6")
 
     out = {
         "metric": METRIC,
@@ -453,7 +460,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.config, kernel),
                      "kernel": kernel, "bytes_per_update": B,
                      "updates_per_launch": tm["updates_per_launch"],
-                     "mean_launch_us": ms_launch * 1e3},
+                     "mean_launch_us": ms_launch * 1e3, "copy_bandwidth_GBs": copy_gbs,
+                     "frac_of_copy": achieved / copy_gbs,
+                     "fp64_flops_per_update": flops, "fp64_frac": value * flops / 78.6e12},
         "cpu_baseline": cpu,
         "acceptance": acc_rate,
         "swap_acceptance": swap_rate,
