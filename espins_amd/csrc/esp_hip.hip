@@ -44,18 +44,30 @@ using PhaseFn = void (*)(KArgs);
 using LdsFn = void (*)(KArgs, int, LdsMeas);
 using EnergyFn = void (*)(KArgs, int, double *);
 
-template <int MODE, bool CMP, bool HB, bool HD>
-PhaseFn ph4(bool self) { return self ? k_phase<MODE, CMP, HB, HD, true> : k_phase<MODE, CMP, HB, HD, false>; }
-template <int MODE, bool CMP, bool HB>
-PhaseFn ph3(bool hd, bool self) { return hd ? ph4<MODE, CMP, HB, true>(self) : ph4<MODE, CMP, HB, false>(self); }
-template <int MODE, bool CMP>
-PhaseFn ph2(bool hb, bool hd, bool self) { return hb ? ph3<MODE, CMP, true>(hd, self) : ph3<MODE, CMP, false>(hd, self); }
-template <int MODE>
-PhaseFn ph1(bool cmp, bool hb, bool hd, bool self) { return cmp ? ph2<MODE, true>(hb, hd, self) : ph2<MODE, false>(hb, hd, self); }
-PhaseFn phase_fn(int mode, const esp::HostModel &m) {
-  if (mode == M_RANDOM) return ph1<M_RANDOM>(m.compressed, m.has_B, m.has_D, m.has_self);
-  if (mode == M_CONST) return ph1<M_CONST>(m.compressed, m.has_B, m.has_D, m.has_self);
-  return ph1<M_HOSTFED>(m.compressed, m.has_B, m.has_D, m.has_self);
+template <int MODE, bool CMP, bool HB, bool HD, int CH>
+PhaseFn ph4(bool self) { return self ? k_phase<MODE, CMP, HB, HD, true, CH> : k_phase<MODE, CMP, HB, HD, false, CH>; }
+template <int MODE, bool CMP, bool HB, int CH>
+PhaseFn ph3(bool hd, bool self) { return hd ? ph4<MODE, CMP, HB, true, CH>(self) : ph4<MODE, CMP, HB, false, CH>(self); }
+template <int MODE, bool CMP, int CH>
+PhaseFn ph2(bool hb, bool hd, bool self) { return hb ? ph3<MODE, CMP, true, CH>(hd, self) : ph3<MODE, CMP, false, CH>(hd, self); }
+template <int MODE, int CH>
+PhaseFn ph1(bool cmp, bool hb, bool hd, bool self) { return cmp ? ph2<MODE, true, CH>(hb, hd, self) : ph2<MODE, false, CH>(hb, hd, self); }
+template <int CH>
+PhaseFn phase_fn_ch(int mode, const esp::HostModel &m) {
+  if (mode == M_RANDOM) return ph1<M_RANDOM, CH>(m.compressed, m.has_B, m.has_D, m.has_self);
+  if (mode == M_CONST) return ph1<M_CONST, CH>(m.compressed, m.has_B, m.has_D, m.has_self);
+  return ph1<M_HOSTFED, CH>(m.compressed, m.has_B, m.has_D, m.has_self);
+}
+// Lanes of one colour phase of every replica (mean over colours).
+int64_t launch_lanes(const esp_ctx *c) {
+  return (int64_t)c->hm.N * c->tps / std::max(1, c->hm.ncol) * c->R;
+}
+
+// Wide chunks (more gathers in flight per lane, more registers) when one colour phase of all
+// replicas has too few waves to hide memory latency by occupancy: < 4 waves per SIMD.
+PhaseFn phase_fn(int mode, const esp::HostModel &m, int64_t lanes_per_launch) {
+  const bool wide = lanes_per_launch < 4 * 1024 * 64;
+  return wide ? phase_fn_ch<kChunkWide>(mode, m) : phase_fn_ch<kChunk>(mode, m);
 }
 template <int MODE, bool CMP, bool HB, bool HD>
 LdsFn ld4(bool self) { return self ? k_lds<MODE, CMP, HB, HD, true> : k_lds<MODE, CMP, HB, HD, false>; }
@@ -86,22 +98,33 @@ int check_ready(esp_ctx *c, bool need_replicas) {
   return ESP_OK;
 }
 
+// Kernel arguments of the colour phases.
+KArgs phase_args(esp_ctx *c, double tilt, uint64_t seed) {
+  KArgs a = base_args(c);
+  a.tilt = tilt;
+  a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32);
+  a.xcd_remap = (c->R % 8 == 0) ? 1 : 0;
+  return a;
+}
+
+// Colour `col` of `a` (block count for the lanes per site).
+void set_colour(KArgs &a, const esp::HostModel &m, int col) {
+  a.c_begin = m.col_off[col];
+  a.c_count = m.col_off[col + 1] - m.col_off[col];
+  a.nblk = (a.c_count * a.tps + kBlock - 1) / kBlock;
+}
+
 // Enqueue `nsweeps` sweeps starting at clock->sweep + off0 (global-memory path).
 int enqueue_phase_sweeps(esp_ctx *c, int mode, double tilt, uint64_t seed, int nsweeps, int off0) {
   if (mode == ESP_MODE_LEARNED) return esp::learn_enqueue_sweeps(c, seed, nsweeps, off0);
   const esp::HostModel &m = c->hm;
-  KArgs a = base_args(c);
-  a.tilt = tilt;
-  a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32);
-  PhaseFn fn = phase_fn(mode, m);
+  KArgs a = phase_args(c, tilt, seed);
+  PhaseFn fn = phase_fn(mode, m, launch_lanes(c));
   const size_t shm = m.compressed ? c->ctab_bytes : 0;
   for (int s = 0; s < nsweeps; ++s)
     for (int col = 0; col < m.ncol; ++col) {
-      a.c_begin = m.col_off[col];
-      a.c_count = m.col_off[col + 1] - m.col_off[col];
+      set_colour(a, m, col);
       if (a.c_count == 0) continue;
-      a.nblk = (a.c_count * a.tps + kBlock - 1) / kBlock;
-      a.xcd_remap = (c->R % 8 == 0) ? 1 : 0;
       a.sweep_off = off0 + s;
       hipLaunchKernelGGL(fn, dim3(a.nblk * c->R), dim3(kBlock), shm, c->stream, a);
     }
@@ -560,7 +583,7 @@ int esp_sweep_hostfed(esp_ctx *c, const double *prop, const double *unif, uint8_
   HIPCHK(c, hipMemcpyAsync(c->d_unif, unif, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   KArgs a = base_args(c);
   a.prop = c->d_prop; a.unif = c->d_unif; a.accepted = c->d_accd;
-  PhaseFn fn = phase_fn(M_HOSTFED, m);
+  PhaseFn fn = phase_fn(M_HOSTFED, m, launch_lanes(c));
   if ((rc = set_clock(c, 0, 0, 1))) return rc;
   for (int col = 0; col < m.ncol; ++col) {
     a.c_begin = m.col_off[col];
@@ -672,17 +695,12 @@ int esp_time_sweeps(esp_ctx *c, int nsweeps, int mode, double tilt, uint64_t see
     HIPCHK(c, hipGetLastError());
     k = 1;
   } else {
-    KArgs a = base_args(c);
-    a.tilt = tilt;
-    a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32);
-    PhaseFn fn = phase_fn(mode, m);
+    KArgs a = phase_args(c, tilt, seed);
+    PhaseFn fn = phase_fn(mode, m, launch_lanes(c));
     for (int sw = 0; sw < nsweeps; ++sw)
       for (int col = 0; col < m.ncol; ++col) {
-        a.c_begin = m.col_off[col];
-        a.c_count = m.col_off[col + 1] - m.col_off[col];
+        set_colour(a, m, col);
         if (a.c_count == 0) continue;
-        a.nblk = (a.c_count * a.tps + kBlock - 1) / kBlock;
-        a.xcd_remap = (c->R % 8 == 0) ? 1 : 0;
         a.sweep_off = sw;
         // start/stop stamped by the dispatch packet itself
         hipExtLaunchKernelGGL(fn, dim3(a.nblk * c->R), dim3(kBlock),

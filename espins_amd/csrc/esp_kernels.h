@@ -26,6 +26,7 @@ using namespace espd;
 constexpr int kBlock = 256;      // phase / reduce kernels (4 waves)
 constexpr int kLdsBlock = 512;   // LDS-resident kernel (8 waves per replica)
 constexpr int kChunk = 8;        // partner gathers issued together per lane
+constexpr int kChunkWide = 16;   // ... for launches too small to fill the GPU with waves
 // Dynamic LDS of the LDS-resident kernel: [N+1 spins as double4][coefficient table]
 // [kLdsRed doubles of reduction scratch]; every byte of LDS it uses is in that one region.
 constexpr int kLdsRed = 4 * (512 / 64);
@@ -135,10 +136,12 @@ __device__ __forceinline__ void bond_term(const KArgs &a, const double *crow, in
 }
 
 // Partial bond sums over the slots l = g, g+tps, ... of site d.  Inactive lanes return
-// zeros.  Full chunks issue kChunk index loads, then kChunk partner gathers, then the
-// arithmetic, with no masks; only the last partial chunk is masked.  Padded slots point at
-// the zero spin (index N), so they need no test; a self bond (SELF) reads the new spin.
-template <bool CMP, bool HB, bool HD, bool SELF, class SP>
+// zeros.  Slots go in chunks of CH: the CH partner indices of the next chunk are loaded while
+// the current chunk's CH partner gathers are in flight, so a wave waits for one index round
+// trip plus one gather round trip per chunk.  Full chunks need no masks; only the last
+// partial chunk is masked (its spare lanes re-read slot g).  Padded slots point at the zero
+// spin (index N), so they need no test; a self bond (SELF) reads the new spin.
+template <int CH, bool CMP, bool HB, bool HD, bool SELF, class SP>
 __device__ __forceinline__ void bond_sums(const KArgs &a, const SP &sp, const double *ctab, int d,
                                           bool active, int g, const Proposal &p, double &eJ,
                                           double &eB, double &eD) {
@@ -148,39 +151,39 @@ __device__ __forceinline__ void bond_sums(const KArgs &a, const SP &sp, const do
   const V3 si = sub(p.sn, ds);  // old spin as the reference recovers it (get_quant.F90:306,320)
   const int tps = a.tps;
   const int nk = (a.zmax - g + tps - 1) / tps;
-  const int nfull = nk - nk % kChunk;
+  const int nfull = nk - nk % CH;
   const double *crow = CMP ? ctab + (size_t)a.basis[d] * a.zmax * 5 : nullptr;
   const unsigned ud = (unsigned)d;
-  for (int k0 = 0; k0 < nfull; k0 += kChunk) {
-    unsigned jj[kChunk];
-    V3 sv[kChunk];
+  unsigned jn[CH];   // partner indices of the next chunk
+  auto load_idx = [&](int k0) {
 #pragma unroll
-    for (int u = 0; u < kChunk; ++u)
-      jj[u] = (unsigned)ld_off(a.nbr, ((unsigned)(g + tps * (k0 + u)) * (unsigned)a.N + ud) * 4u);
-#pragma unroll
-    for (int u = 0; u < kChunk; ++u) {
-      const V3 t = sp.ld(jj[u]);
-      sv[u] = (SELF && jj[u] == ud) ? p.sn : t;
+    for (int u = 0; u < CH; ++u) {
+      const int l = k0 + u < nk ? g + tps * (k0 + u) : g;
+      jn[u] = (unsigned)ld_off(a.nbr, ((unsigned)l * (unsigned)a.N + ud) * 4u);
     }
+  };
+  if (nk > 0) load_idx(0);
+  for (int k0 = 0; k0 < nfull; k0 += CH) {
+    V3 sv[CH];
 #pragma unroll
-    for (int u = 0; u < kChunk; ++u) bond_term<CMP, HB, HD>(a, crow, g + tps * (k0 + u), d, ds, si, sv[u], eJ, eB, eD);
+    for (int u = 0; u < CH; ++u) {
+      const V3 t = sp.ld(jn[u]);
+      sv[u] = (SELF && jn[u] == ud) ? p.sn : t;
+    }
+    if (k0 + CH < nk) load_idx(k0 + CH);
+#pragma unroll
+    for (int u = 0; u < CH; ++u) bond_term<CMP, HB, HD>(a, crow, g + tps * (k0 + u), d, ds, si, sv[u], eJ, eB, eD);
   }
   const int rem = nk - nfull;
   if (rem > 0) {
-    unsigned jj[kChunk];
-    V3 sv[kChunk];
+    V3 sv[CH];
 #pragma unroll
-    for (int u = 0; u < kChunk; ++u) {
-      const int l = u < rem ? g + tps * (nfull + u) : g;
-      jj[u] = (unsigned)ld_off(a.nbr, ((unsigned)l * (unsigned)a.N + ud) * 4u);
+    for (int u = 0; u < CH; ++u) {
+      const V3 t = sp.ld(jn[u]);
+      sv[u] = (SELF && jn[u] == ud) ? p.sn : t;
     }
 #pragma unroll
-    for (int u = 0; u < kChunk; ++u) {
-      const V3 t = sp.ld(jj[u]);
-      sv[u] = (SELF && jj[u] == ud) ? p.sn : t;
-    }
-#pragma unroll
-    for (int u = 0; u < kChunk; ++u)
+    for (int u = 0; u < CH; ++u)
       if (u < rem) bond_term<CMP, HB, HD>(a, crow, g + tps * (nfull + u), d, ds, si, sv[u], eJ, eB, eD);
   }
 }
@@ -229,7 +232,7 @@ __device__ __forceinline__ void decode_block(const KArgs &a, int &r, int &b) {
 
 // Block b of colour phase (a.c_begin, a.c_count) of replica r (global-memory path);
 // a.tps lanes per site.  Shared by the single-context and the batched kernels.
-template <int MODE, bool CMP, bool HB, bool HD, bool SELF>
+template <int MODE, bool CMP, bool HB, bool HD, bool SELF, int CH = kChunk>
 __device__ __forceinline__ void phase_body(const KArgs &a, int r, int b, double *s_ctab) {
   if (CMP) {
     const int n = a.nb * a.zmax * 5;
@@ -249,7 +252,7 @@ __device__ __forceinline__ void phase_body(const KArgs &a, int r, int b, double 
   Proposal p{};
   if (active) p = propose<MODE>(a, sp, d, r, sweep);
   double eJ, eB, eD;
-  bond_sums<CMP, HB, HD, SELF>(a, sp, s_ctab, d, active, g, p, eJ, eB, eD);
+  bond_sums<CH, CMP, HB, HD, SELF>(a, sp, s_ctab, d, active, g, p, eJ, eB, eD);
   eJ = group_sum(eJ, tps);
   if (HB) eB = group_sum(eB, tps);
   if (HD) eD = group_sum(eD, tps);
@@ -270,13 +273,13 @@ __device__ __forceinline__ void phase_body(const KArgs &a, int r, int b, double 
 }
 
 // One colour phase over every replica of one context.
-template <int MODE, bool CMP, bool HB, bool HD, bool SELF>
+template <int MODE, bool CMP, bool HB, bool HD, bool SELF, int CH>
 __global__ __launch_bounds__(kBlock) void k_phase(KArgs a) {
   extern __shared__ double s_ctab[];
   int r, b;
   decode_block(a, r, b);
   if (r >= a.R) return;  // uniform per block
-  phase_body<MODE, CMP, HB, HD, SELF>(a, r, b, s_ctab);
+  phase_body<MODE, CMP, HB, HD, SELF, CH>(a, r, b, s_ctab);
 }
 
 struct LdsMeas {
@@ -330,7 +333,7 @@ __device__ __forceinline__ void lds_body(const KArgs &a, int r, int nsweeps, con
         Proposal p{};
         if (active) p = propose<MODE>(a, sp, d, r, sweep);
         double eJ, eB, eD;
-        bond_sums<CMP, HB, HD, SELF>(a, sp, ctab, d, active, g, p, eJ, eB, eD);
+        bond_sums<kChunk, CMP, HB, HD, SELF>(a, sp, ctab, d, active, g, p, eJ, eB, eD);
         eJ = group_sum(eJ, tps);
         if (HB) eB = group_sum(eB, tps);
         if (HD) eD = group_sum(eD, tps);
