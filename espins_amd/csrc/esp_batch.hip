@@ -34,6 +34,7 @@ __global__ __launch_bounds__(kBlock) void k_bphase(const KArgs *chains, const in
   const int4 t = tiles[blockIdx.x];
   if (t.x < 0) return;
   KArgs a = chains[t.x];
+  a.col = colour;
   a.c_begin = a.col_off[colour];
   a.c_count = a.col_off[colour + 1] - a.c_begin;
   a.sweep_off = sweep_off;
@@ -51,27 +52,16 @@ __global__ __launch_bounds__(kLdsBlock) void k_blds(const KArgs *chains, const L
   lds_body<MODE, CMP, HB, HD, SELF>(chains[it.x], it.y, nsweeps, m, s_sp);
 }
 
-// partial (w, x, y, z) sums of global-path replicas: grid (max blocks, items)
+// partial (w, x, y, z) sums of global-path replicas: grid (max blocks, items).  Every round
+// sweeps all colours first, so the spin sums come from the colour phases' per-wave sums.
 __global__ __launch_bounds__(kBlock) void k_breduce(const KArgs *chains, const int *red_blocks,
                                                     double *const *partials, const int2 *items) {
-  __shared__ double red[4 * (kBlock / 64)];
   const int2 it = items[blockIdx.y];
   const int nblk = red_blocks[it.x];
   const int b = blockIdx.x;
-  if (b >= nblk) return;
+  if (b >= nblk) return;   // uniform per block
   const KArgs &a = chains[it.x];
-  const int N = a.N, Ns = a.Ns, r = it.y;
-  const int chunk = (N + nblk - 1) / nblk;
-  const int beg = b * chunk, end = min(N, beg + chunk);
-  double v[4] = {0, 0, 0, 0};
-  const double *x = a.sp + (size_t)r * 3 * Ns;
-  const double *wr = a.w + (size_t)r * Ns;
-  for (int d = beg + threadIdx.x; d < end; d += kBlock) {
-    v[0] += wr[d]; v[1] += x[d]; v[2] += x[Ns + d]; v[3] += x[2 * Ns + d];
-  }
-  block_sum<4>(v, red);
-  if (threadIdx.x == 0)
-    for (int k = 0; k < 4; ++k) partials[it.x][((size_t)r * nblk + b) * 4 + k] = v[k];
+  reduce_body(a.sp, a.w, a.N, a.Ns, nblk, partials[it.x], a.mpart, a.ncol * a.mpart_nw, it.y, b);
 }
 
 // one block per chain; the last block to finish advances the shared clock
@@ -156,6 +146,8 @@ struct Chain {
       *slot_rep = nullptr, *rep_slot = nullptr;
   uint8_t *basis = nullptr;
   unsigned long long *rej = nullptr, *nswap = nullptr, *nmeas = nullptr;
+  double *mpart = nullptr;   // per-wave spin sums of the colour phases (global-path chains)
+  int mpart_nw = 0;
 };
 
 }  // namespace
@@ -238,6 +230,8 @@ void free_chain(Chain &c) {
   bfree(c.partial); bfree(c.cJ); bfree(c.cB); bfree(c.cD); bfree(c.ctab); bfree(c.mag);
   bfree(c.si4); bfree(c.h4); bfree(c.nbr); bfree(c.dev2ref); bfree(c.ref2dev); bfree(c.col_off);
   bfree(c.slot_rep); bfree(c.rep_slot); bfree(c.basis); bfree(c.rej); bfree(c.nswap); bfree(c.nmeas);
+  bfree(c.mpart);
+  c.mpart_nw = 0;
 }
 
 void drop_plan(esp_batch *b) {
@@ -278,7 +272,23 @@ int build_plan(esp_batch *b, double tilt, uint64_t seed) {
   b->max_red_blocks = 1;
   for (int k = 0; k < nc; ++k) {
     Chain &c = b->chains[k];
+    if (!c.lds) {   // per-wave spin sums for this plan's lanes per site, zeroed
+      int nw = 1;
+      for (int col = 0; col < c.hm.ncol; ++col)
+        nw = std::max(nw, ((c.hm.col_off[col + 1] - c.hm.col_off[col]) * c.tps + 63) / 64);
+      const size_t n = (size_t)c.R * c.hm.ncol * nw * 3;
+      if (nw != c.mpart_nw || !c.mpart) {
+        bfree(c.mpart);
+        if (hipMalloc((void **)&c.mpart, n * sizeof(double)) != hipSuccess) {
+          c.mpart = nullptr;
+          return bfail(b, ESP_ENOMEM, "hipMalloc failed");
+        }
+        c.mpart_nw = nw;
+      }
+      BCHK(b, hipMemsetAsync(c.mpart, 0, n * sizeof(double), b->stream));
+    }
     ka[k] = chain_args(c, k);
+    if (!c.lds) { ka[k].mpart = c.mpart; ka[k].mpart_nw = c.mpart_nw; }
     ka[k].clock = b->d_clock;
     ka[k].tilt = tilt;
     ka[k].key0 = (uint32_t)seed; ka[k].key1 = (uint32_t)(seed >> 32);
@@ -823,7 +833,8 @@ int esp_batch_get_stats(esp_batch *b, int k, double *acc, int64_t *rejected, dou
     // fresh running energies: partial sums over the replica, e_base + sum w
     KArgs a = chain_args(c, k);
     hipLaunchKernelGGL(k_reduce, dim3(c.red_blocks, c.R), dim3(kBlock), 0, b->stream,
-                       (const double *)a.sp, (const double *)a.w, a.N, a.Ns, c.red_blocks, c.partial);
+                       (const double *)a.sp, (const double *)a.w, a.N, a.Ns, c.red_blocks, c.partial,
+                       (const double *)nullptr, 0);
     FinArgs f{};
     f.partial = c.partial; f.nblk = c.red_blocks; f.N = c.hm.N; f.R = c.R;
     f.e_base = c.ebase; f.e_run = c.erun; f.acc = c.acc; f.nmeas = c.nmeas;

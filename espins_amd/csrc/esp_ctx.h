@@ -48,6 +48,11 @@ struct esp_ctx {
   // host-fed staging
   double *d_prop = nullptr, *d_unif = nullptr;
   uint8_t *d_accd = nullptr;
+  // per-wave spin sums of the colour phases: [R][ncol][mpart_nw][3]; valid while the last
+  // change of the spins was a full global-path sweep
+  double *d_mpart = nullptr;
+  int mpart_nw = 0;
+  bool mpart_valid = false;
   // per-measurement trace of E and |m| by slot (global-memory path only)
   double *d_trace = nullptr;      // [trace_cap][R][2]
   int trace_cap = 0;
@@ -121,6 +126,9 @@ void free_replicas(esp_ctx *c) {
   dfree(c->d_nswap); dfree(c->d_nmeas); dfree(c->d_prop); dfree(c->d_unif); dfree(c->d_accd);
   dfree(c->d_trace);
   c->trace_cap = 0;
+  dfree(c->d_mpart);
+  c->mpart_nw = 0;
+  c->mpart_valid = false;
   c->R = 0;
   drop_graph(c);
 }

@@ -90,25 +90,50 @@ EnergyFn energy_fn(bool cmp, bool hb, bool hd) {
   return hd ? k_energy<false, false, true> : k_energy<false, false, false>;
 }
 
+int ensure_mpart(esp_ctx *c);
+
 int check_ready(esp_ctx *c, bool need_replicas) {
   if (!c) return ESP_EINVAL;
   if (!c->have_model) return fail(c, ESP_ESTATE, "no model: call esp_set_model first");
   if (need_replicas && c->R <= 0) return fail(c, ESP_ESTATE, "no replicas: call esp_set_replicas first");
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, ESP_EHIP, "hipSetDevice failed");
+  return need_replicas ? ensure_mpart(c) : ESP_OK;
+}
+
+// Per-wave spin sums written by the colour phases, sized for the current lanes per site
+// and zeroed on (re)allocation: entries of waves a colour does not have stay zero.  Called
+// before any capture, never inside one.
+int ensure_mpart(esp_ctx *c) {
+  const esp::HostModel &m = c->hm;
+  int need = 1;
+  for (int col = 0; col < m.ncol; ++col)
+    need = std::max(need, ((m.col_off[col + 1] - m.col_off[col]) * c->tps + 63) / 64);
+  if (c->d_mpart && need == c->mpart_nw) return ESP_OK;
+  drop_graph(c);   // a captured round may hold the old buffer
+  const size_t n = (size_t)c->R * m.ncol * need * 3;
+  int rc = dalloc(c, c->d_mpart, n);
+  if (rc) return rc;
+  HIPCHK(c, hipMemsetAsync(c->d_mpart, 0, n * sizeof(double), c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->mpart_nw = need;
+  c->mpart_valid = false;
   return ESP_OK;
 }
 
-// Kernel arguments of the colour phases.
+// Kernel arguments of the colour phases, with the per-wave spin sums attached.
 KArgs phase_args(esp_ctx *c, double tilt, uint64_t seed) {
   KArgs a = base_args(c);
   a.tilt = tilt;
   a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32);
   a.xcd_remap = (c->R % 8 == 0) ? 1 : 0;
+  a.mpart = c->d_mpart;
+  a.mpart_nw = c->mpart_nw;
   return a;
 }
 
 // Colour `col` of `a` (block count for the lanes per site).
 void set_colour(KArgs &a, const esp::HostModel &m, int col) {
+  a.col = col;
   a.c_begin = m.col_off[col];
   a.c_count = m.col_off[col + 1] - m.col_off[col];
   a.nblk = (a.c_count * a.tps + kBlock - 1) / kBlock;
@@ -116,7 +141,10 @@ void set_colour(KArgs &a, const esp::HostModel &m, int col) {
 
 // Enqueue `nsweeps` sweeps starting at clock->sweep + off0 (global-memory path).
 int enqueue_phase_sweeps(esp_ctx *c, int mode, double tilt, uint64_t seed, int nsweeps, int off0) {
-  if (mode == ESP_MODE_LEARNED) return esp::learn_enqueue_sweeps(c, seed, nsweeps, off0);
+  if (mode == ESP_MODE_LEARNED) {
+    c->mpart_valid = false;
+    return esp::learn_enqueue_sweeps(c, seed, nsweeps, off0);
+  }
   const esp::HostModel &m = c->hm;
   KArgs a = phase_args(c, tilt, seed);
   PhaseFn fn = phase_fn(mode, m, launch_lanes(c));
@@ -129,6 +157,7 @@ int enqueue_phase_sweeps(esp_ctx *c, int mode, double tilt, uint64_t seed, int n
       hipLaunchKernelGGL(fn, dim3(a.nblk * c->R), dim3(kBlock), shm, c->stream, a);
     }
   HIPCHK(c, hipGetLastError());
+  if (nsweeps > 0) c->mpart_valid = a.mpart != nullptr;
   return ESP_OK;
 }
 
@@ -142,13 +171,18 @@ int enqueue_lds_sweeps(esp_ctx *c, int mode, double tilt, uint64_t seed, int nsw
   LdsFn fn = lds_fn(mode, m);
   hipLaunchKernelGGL(fn, dim3(c->R), dim3(kLdsBlock), c->lds_bytes, c->stream, a, nsweeps, lm);
   HIPCHK(c, hipGetLastError());
+  c->mpart_valid = false;
   return ESP_OK;
 }
 
+// Running-energy and magnetisation sums: the spins' sums come from the colour phases'
+// per-wave sums when the last sweep left them, else from the spins.
 int enqueue_reduce(esp_ctx *c) {
+  const bool mp = c->mpart_valid;
   hipLaunchKernelGGL(k_reduce, dim3(c->red_blocks, c->R), dim3(kBlock), 0, c->stream,
                      (const double *)c->d_sp, (const double *)c->d_w, c->hm.N, c->Ns,
-                     c->red_blocks, c->d_partial);
+                     c->red_blocks, c->d_partial, mp ? (const double *)c->d_mpart : nullptr,
+                     c->hm.ncol * c->mpart_nw);
   HIPCHK(c, hipGetLastError());
   return ESP_OK;
 }
@@ -198,6 +232,7 @@ int rebase(esp_ctx *c) {
   if (rc) return rc;
   const size_t n = (size_t)c->R * c->Ns;
   hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->d_w, n, 0.0);
+  c->mpart_valid = false;
   HIPCHK(c, hipMemcpyAsync(c->d_erun, c->d_ebase, sizeof(double) * c->R, hipMemcpyDeviceToDevice,
                            c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -581,8 +616,9 @@ int esp_sweep_hostfed(esp_ctx *c, const double *prop, const double *unif, uint8_
   }
   HIPCHK(c, hipMemcpyAsync(c->d_prop, prop, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_unif, unif, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-  KArgs a = base_args(c);
+  KArgs a = base_args(c);   // one lane per site here: no per-wave sums
   a.prop = c->d_prop; a.unif = c->d_unif; a.accepted = c->d_accd;
+  c->mpart_valid = false;
   PhaseFn fn = phase_fn(M_HOSTFED, m, launch_lanes(c));
   if ((rc = set_clock(c, 0, 0, 1))) return rc;
   for (int col = 0; col < m.ncol; ++col) {
@@ -710,6 +746,7 @@ int esp_time_sweeps(esp_ctx *c, int nsweeps, int mode, double tilt, uint64_t see
       }
     HIPCHK(c, hipGetLastError());
   }
+  c->mpart_valid = !lds && mode != ESP_MODE_LEARNED;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   double sum = 0.0;
   for (int q = 0; q < k; ++q) {

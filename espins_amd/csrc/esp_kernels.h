@@ -54,6 +54,10 @@ struct KArgs {
   int sweep_off, tps;            // tps: lanes cooperating on one site (1, 2, 4, 8)
   double tilt;
   uint32_t key0, key1, stream_base;
+  // per-wave sums of the spins after each colour phase, or null: mpart[R][ncol][mpart_nw][3];
+  // phase `col` overwrites its waves' entries
+  double *mpart;
+  int col, mpart_nw;
 };
 
 enum { M_RANDOM = 0, M_CONST = 1, M_HOSTFED = 2 };
@@ -270,6 +274,24 @@ __device__ __forceinline__ void phase_body(const KArgs &a, int r, int b, double 
   }
   const unsigned long long m = __ballot(rej && g == 0);
   if ((threadIdx.x & 63) == 0 && m) atomicAdd(&a.rej[slot], (unsigned long long)__popcll(m));
+  // A site changes only in its own colour phase, so after a sweep the per-wave spin sums of
+  // its phases add up to the replica's magnetisation: the measurement then reads these
+  // instead of every spin.  Wave-local (fixed butterfly, one writer per entry): no barrier.
+  if (a.mpart) {
+    V3 sf = {0.0, 0.0, 0.0};
+    if (active && g == 0) sf = rej ? p.so : p.sn;
+    for (int o = 1; o < 64; o <<= 1) {
+      sf.x += __shfl_xor(sf.x, o, 64);
+      sf.y += __shfl_xor(sf.y, o, 64);
+      sf.z += __shfl_xor(sf.z, o, 64);
+    }
+    // waves past the colour's last lane own no entry (mpart_nw counts the colour's waves)
+    const int wave = (b * kBlock + threadIdx.x) >> 6;
+    if ((threadIdx.x & 63) == 0 && wave * 64 < a.c_count * tps) {
+      double *o = a.mpart + (((size_t)r * a.ncol + a.col) * a.mpart_nw + wave) * 3;
+      o[0] = sf.x; o[1] = sf.y; o[2] = sf.z;
+    }
+  }
 }
 
 // One colour phase over every replica of one context.
@@ -398,22 +420,37 @@ __global__ __launch_bounds__(kLdsBlock) void k_lds(KArgs a, int nsweeps, LdsMeas
   lds_body<MODE, CMP, HB, HD, SELF>(a, blockIdx.x, nsweeps, m, s_sp);
 }
 
-// Partial sums of (w, x, y, z) per (replica, block) for the running energy and m.
-__global__ __launch_bounds__(kBlock) void k_reduce(const double *sp, const double *w, int N, int Ns,
-                                                   int nblk, double *partial) {
+// Partial sums of (w, x, y, z) per (replica, block) for the running energy and m.  With
+// mpart (the colour phases' per-wave spin sums, nm = ncol * mpart_nw entries per replica)
+// the spins themselves are not read.
+__device__ __forceinline__ void reduce_body(const double *sp, const double *w, int N, int Ns, int nblk,
+                                            double *partial, const double *mpart, int nm, int r, int b) {
   __shared__ double red[4 * (kBlock / 64)];
-  const int r = blockIdx.y, b = blockIdx.x;
   const int chunk = (N + nblk - 1) / nblk;
   const int beg = b * chunk, end = min(N, beg + chunk);
   double v[4] = {0, 0, 0, 0};
   const double *x = sp + (size_t)r * 3 * Ns;
   const double *wr = w + (size_t)r * Ns;
-  for (int d = beg + threadIdx.x; d < end; d += kBlock) {
-    v[0] += wr[d]; v[1] += x[d]; v[2] += x[Ns + d]; v[3] += x[2 * Ns + d];
+  if (mpart) {
+    for (int d = beg + threadIdx.x; d < end; d += kBlock) v[0] += wr[d];
+    const int mc = (nm + nblk - 1) / nblk;
+    const double *mr = mpart + (size_t)r * nm * 3;
+    for (int q = b * mc + threadIdx.x; q < min(nm, (b + 1) * mc); q += kBlock) {
+      v[1] += mr[3 * q]; v[2] += mr[3 * q + 1]; v[3] += mr[3 * q + 2];
+    }
+  } else {
+    for (int d = beg + threadIdx.x; d < end; d += kBlock) {
+      v[0] += wr[d]; v[1] += x[d]; v[2] += x[Ns + d]; v[3] += x[2 * Ns + d];
+    }
   }
   block_sum<4>(v, red);
   if (threadIdx.x == 0)
     for (int k = 0; k < 4; ++k) partial[((size_t)r * nblk + b) * 4 + k] = v[k];
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce(const double *sp, const double *w, int N, int Ns,
+                                                   int nblk, double *partial, const double *mpart, int nm) {
+  reduce_body(sp, w, N, Ns, nblk, partial, mpart, nm, blockIdx.y, blockIdx.x);
 }
 
 struct FinArgs {
