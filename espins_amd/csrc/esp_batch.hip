@@ -757,18 +757,47 @@ int time_chain(esp_batch *b, int k, bool lds, int tps, int nsweeps, int mode, do
 }
 }  // namespace
 
+// Mean time of one round (one sweep of every chain + exchange) of the current plan, after one
+// untimed round.  Advances the chains; the tuning clock keeps its own Philox counters.
+int time_rounds(esp_batch *b, int nrounds, double *ms) {
+  int rc = build_plan(b, 0.125, 0x74756e65ull);
+  if (rc) return rc;
+  Clock h{(1ull << 61), 0, 1, 0};
+  BCHK(b, hipMemcpyAsync(b->d_clock, &h, sizeof(Clock), hipMemcpyHostToDevice, b->stream));
+  if ((rc = enqueue_round(b, 1, 1, false, true, ESP_MODE_RANDOM))) return rc;
+  hipEvent_t e0, e1;
+  BCHK(b, hipEventCreate(&e0));
+  BCHK(b, hipEventCreate(&e1));
+  BCHK(b, hipEventRecord(e0, b->stream));
+  for (int k = 0; k < nrounds && rc == ESP_OK; ++k) rc = enqueue_round(b, 1, 1, false, true, ESP_MODE_RANDOM);
+  BCHK(b, hipEventRecord(e1, b->stream));
+  BCHK(b, hipStreamSynchronize(b->stream));
+  float t = 0.f;
+  BCHK(b, hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = t / nrounds;
+  return rc;
+}
+
 int esp_batch_autotune(esp_batch *b, int nsweeps, const char *cache_path) {
+  int rc_reset;
   if (!b) return ESP_EINVAL;
   if (!b->committed) return bfail(b, ESP_ESTATE, "commit the batch before tuning");
   if (nsweeps < 1) return bfail(b, ESP_EINVAL, "nsweeps must be >= 1");
   BCHK(b, hipSetDevice(b->device));
   std::map<std::string, std::pair<int, int>> decided;  // key -> (lds, tps)
+  std::map<std::string, int> lds_tps;                   // key -> best lanes per site in LDS
   if (cache_path) {
     if (FILE *f = std::fopen(cache_path, "r")) {
-      char key[128];
-      int l, t;
+      char line[256], key[128];
+      int l, t, lt;
       double ms;
-      while (std::fscanf(f, "%127s %d %d %lf", key, &l, &t, &ms) == 4) decided[key] = {l, t};
+      while (std::fgets(line, sizeof line, f)) {
+        const int n = std::sscanf(line, "%127s %d %d %lf %d", key, &l, &t, &ms, &lt);
+        if (n >= 4) decided[key] = {l, t};
+        if (n == 5 && lt > 0) lds_tps[key] = lt;
+      }
       std::fclose(f);
     }
   }
@@ -779,8 +808,8 @@ int esp_batch_autotune(esp_batch *b, int nsweeps, const char *cache_path) {
     const Chain &c = b->chains[kv.second];
     auto it = decided.find(kv.first);
     if (it != decided.end() && (!it->second.first || c.lds_fits)) continue;
-    double best = 1e300;
-    int bl = 0, bt = 1;
+    double best = 1e300, best_lds = 1e300;
+    int bl = 0, bt = 1, blt = 0;
     for (int lds = 0; lds < 2; ++lds) {
       if (lds && !c.lds_fits) continue;
       for (int tps = 1; tps <= 4; tps *= 2) {
@@ -789,9 +818,11 @@ int esp_batch_autotune(esp_batch *b, int nsweeps, const char *cache_path) {
         int rc = time_chain(b, kv.second, lds, tps, nsweeps, ESP_MODE_RANDOM, &ms);
         if (rc) return rc;
         if (ms < best) { best = ms; bl = lds; bt = tps; }
+        if (lds && ms < best_lds) { best_lds = ms; blt = tps; }
       }
     }
     decided[kv.first] = {bl, bt};
+    if (blt) lds_tps[kv.first] = blt;
     best_ms[kv.first] = best;
   }
   for (auto &c : b->chains) {
@@ -802,11 +833,58 @@ int esp_batch_autotune(esp_batch *b, int nsweeps, const char *cache_path) {
   if (cache_path) {
     if (FILE *f = std::fopen(cache_path, "w")) {
       for (auto &kv : decided)
-        std::fprintf(f, "%s %d %d %.6f\n", kv.first.c_str(), kv.second.first, kv.second.second,
-                     best_ms.count(kv.first) ? best_ms[kv.first] : 0.0);
+        std::fprintf(f, "%s %d %d %.6f %d\n", kv.first.c_str(), kv.second.first, kv.second.second,
+                     best_ms.count(kv.first) ? best_ms[kv.first] : 0.0,
+                     lds_tps.count(kv.first) ? lds_tps[kv.first] : 0);
       std::fclose(f);
     }
   }
+  // Branch balance.  The LDS-resident chains run beside the global-path chains (a forked
+  // graph branch), so a round lasts as long as the longer branch.  Chains that fit in LDS
+  // but ran faster alone on the global path are moved over, largest first, while the
+  // measured round time improves.
+  {
+    std::vector<int> cand;
+    for (int k = 0; k < (int)b->chains.size(); ++k)
+      if (b->chains[k].lds_fits && !b->chains[k].lds) cand.push_back(k);
+    std::sort(cand.begin(), cand.end(), [&](int x, int y) {
+      const Chain &p = b->chains[x], &q = b->chains[y];
+      return (int64_t)p.hm.N * p.R * p.hm.zmax > (int64_t)q.hm.N * q.R * q.hm.zmax;
+    });
+    if (!cand.empty()) {
+      double base = 0;
+      int rc = time_rounds(b, 3, &base);
+      if (rc) return rc;
+      size_t moved = 0, step = 1;
+      std::vector<int> old_tps(cand.size());
+      while (moved < cand.size()) {
+        const size_t m = std::min(cand.size(), moved + step);
+        for (size_t i = moved; i < m; ++i) {
+          Chain &c = b->chains[cand[i]];
+          old_tps[i] = c.tps;
+          c.lds = 1;
+          const auto it = lds_tps.find(shape_key(c));
+          c.tps = it != lds_tps.end() ? it->second : 1;
+        }
+        double t = 0;
+        if ((rc = time_rounds(b, 3, &t))) return rc;
+        if (t < 0.995 * base) {
+          base = t;
+          moved = m;
+          step *= 2;
+        } else {
+          for (size_t i = moved; i < m; ++i) {
+            Chain &c = b->chains[cand[i]];
+            c.lds = 0;
+            c.tps = old_tps[i];
+          }
+          if (step == 1) break;
+          step = 1;
+        }
+      }
+    }
+  }
+  if ((rc_reset = esp_batch_reset_stats(b))) return rc_reset;
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   b->gexec = nullptr;
   b->gkey.clear();
