@@ -273,6 +273,30 @@ std::vector<int> periodic_colouring(int N, int nb, const int L[3],
   return best;
 }
 
+// Smallest period P (per axis, from the candidates of periodic_colouring) such that a site's
+// colour depends only on (basis, cell mod P); P = L when none does.
+void colour_period(int N, int nb, const int L[3], const std::vector<int> &colour, int P[3]) {
+  for (int k = 0; k < 3; ++k) P[k] = L[k];
+  const int cands[] = {1, 2, 3, 4, 6, 8};
+  for (int p : cands) {
+    int Q[3];
+    for (int k = 0; k < 3; ++k) Q[k] = (L[k] % p == 0) ? p : std::gcd(p, L[k]);
+    const int nq = nb * Q[0] * Q[1] * Q[2];
+    std::vector<int> qc(nq, -1);
+    bool ok = true;
+    for (int i = 0; i < N && ok; ++i) {
+      const int c = i / nb, x = c % L[0], y = (c / L[0]) % L[1], z = c / (L[0] * L[1]);
+      const int q = ((z % Q[2] * Q[1] + y % Q[1]) * Q[0] + x % Q[0]) * nb + i % nb;
+      if (qc[q] < 0) qc[q] = colour[i];
+      else ok = qc[q] == colour[i];
+    }
+    if (ok) {
+      for (int k = 0; k < 3; ++k) P[k] = Q[k];
+      return;
+    }
+  }
+}
+
 // Greedy colouring in reference order, then empty the smallest classes where every site
 // can move to a larger class no neighbour uses.
 std::vector<int> greedy_colouring(int N, const std::vector<std::vector<int>> &adj) {
@@ -410,18 +434,35 @@ std::string prepare_model(HostModel &hm, int N, int nb, const double *mag, int z
   for (int c : hm.colour) ncol = std::max(ncol, c + 1);
   hm.ncol = ncol;
 
-  // colour-major device order, reference order inside a colour
+  // Colour-major device order.  Inside a colour, sites are grouped by basis atom and by
+  // residue of the cell modulo the colouring's period, cells in order inside a group: the
+  // lanes of a wave then share the bond geometry of each slot, so their partners are
+  // consecutive sites of one group of the partner colour and a gather touches few cache
+  // lines.  Without a cell grid: by basis atom, then reference order.
   hm.col_off.assign(ncol + 1, 0);
   for (int i = 0; i < N; ++i) hm.col_off[hm.colour[i] + 1]++;
   for (int c = 0; c < ncol; ++c) hm.col_off[c + 1] += hm.col_off[c];
   hm.dev2ref.resize(N);
   hm.ref2dev.resize(N);
   {
-    std::vector<int> fill(hm.col_off.begin(), hm.col_off.end() - 1);
+    std::vector<int64_t> key(N);
+    int P[3] = {1, 1, 1};
+    if (Lc[0] > 0) colour_period(N, nb, Lc, hm.colour, P);
     for (int i = 0; i < N; ++i) {
-      const int d = fill[hm.colour[i]]++;
-      hm.dev2ref[d] = i;
-      hm.ref2dev[i] = d;
+      int64_t res = 0, sub = i / nb;
+      if (Lc[0] > 0) {
+        const int c = i / nb, x = c % Lc[0], y = (c / Lc[0]) % Lc[1], z = c / (Lc[0] * Lc[1]);
+        res = (z % P[2] * P[1] + y % P[1]) * P[0] + x % P[0];
+        sub = ((int64_t)(z / P[2]) * ((Lc[1] + P[1] - 1) / P[1]) + y / P[1]) * ((Lc[0] + P[0] - 1) / P[0]) + x / P[0];
+      }
+      key[i] = (((int64_t)hm.colour[i] * nb + i % nb) * ((int64_t)P[0] * P[1] * P[2]) + res) * (int64_t)N + sub;
+    }
+    std::vector<int> order(N);
+    for (int i = 0; i < N; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int u, int v) { return key[u] < key[v]; });
+    for (int d = 0; d < N; ++d) {
+      hm.dev2ref[d] = order[d];
+      hm.ref2dev[order[d]] = d;
     }
   }
 
