@@ -8,12 +8,14 @@ module espins_hip
   implicit none
   private
 
-  integer(c_int), parameter, public :: ESP_OK = 0, ESP_MODE_RANDOM = 0, ESP_MODE_CONST = 1
+  integer(c_int), parameter, public :: ESP_OK = 0, ESP_MODE_RANDOM = 0, ESP_MODE_CONST = 1, &
+                                       ESP_MODE_LEARNED = 2, ESP_LEARN_NPARAMS = 2502
 
   public :: esp_create, esp_destroy, esp_last_error, esp_version, esp_build_table, &
             esp_set_supercell, esp_set_model, esp_get_model_info, esp_set_replicas, &
             esp_get_spins, esp_put_spins, esp_energy, esp_run, esp_get_stats, esp_get_map, &
-            esp_reset_stats, esp_sync, esp_error_text
+            esp_reset_stats, esp_sync, esp_error_text, esp_sweep, esp_learn_set_weights, &
+            esp_learn_collect, esp_learn_train, esp_learn_clear
 
   interface
     integer(c_int) function esp_create(ctx, device) bind(C, name='esp_create')
@@ -113,6 +115,39 @@ module espins_hip
     integer(c_int) function esp_sync(ctx) bind(C, name='esp_sync')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function esp_sweep(ctx, nsweeps, mode, tilt, seed, sweep0) bind(C, name='esp_sweep')
+      import :: c_int, c_double, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsweeps, mode
+      real(c_double), value :: tilt
+      integer(c_int64_t), value :: seed, sweep0
+    end function
+    ! learned proposal (no counterpart in the reference)
+    integer(c_int) function esp_learn_set_weights(ctx, w, nparams) bind(C, name='esp_learn_set_weights')
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_float), intent(in) :: w(*)
+      integer(c_int), value :: nparams
+    end function
+    integer(c_int) function esp_learn_collect(ctx, n, seed) bind(C, name='esp_learn_collect')
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n
+      integer(c_int64_t), value :: seed
+    end function
+    integer(c_int) function esp_learn_clear(ctx) bind(C, name='esp_learn_clear')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function esp_learn_train(ctx, epochs, batch, lr, seed, epoch_loss) &
+        bind(C, name='esp_learn_train')
+      import :: c_int, c_float, c_double, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: epochs, batch
+      real(c_float), value :: lr
+      integer(c_int64_t), value :: seed
+      real(c_double), intent(out) :: epoch_loss(*)
     end function
   end interface
 

@@ -18,6 +18,7 @@ program mc_gpu
   character(len=256) :: seedname, arg
   integer :: narg, k, i, a, N, R, out_u, mc_u, sc_u, rc, ncell
   integer(c_int) :: zJ, zB, zD, info(6), mode
+  logical :: learned
   integer(c_int), allocatable, target :: jc(:), jn(:), bc(:), bn(:), dc(:), dn(:)
   real(c_double), allocatable, target :: jv(:), bv(:), dvv(:), mag(:), si_par(:), si_ax(:,:), h_par(:), h_ax(:,:)
   real(c_double), allocatable :: spins(:,:,:), tems(:), acc(:,:), e0(:), edir(:), erun(:)
@@ -114,6 +115,7 @@ program mc_gpu
   call check(esp_set_replicas(ctx, int(R, c_int), tems, spins), 'esp_set_replicas')
   call check(esp_energy(ctx, e0), 'esp_energy')
   mode = merge(ESP_MODE_CONST, ESP_MODE_RANDOM, inp%mcarlo_mode(1:5) == 'const')
+  learned = inp%mcarlo_mode(1:5) == 'learn'   ! warm-up with 'random', then train, then learned
 
   if (.not. inp%pt) then
     ! ---- plain MC: warm-up then measured steps, all temperatures at once ----------------
@@ -124,6 +126,7 @@ program mc_gpu
                  0_c_int), 'esp_run warmup')
       sweep = sweep + int(inp%steps_warmup, c_int64_t) * inp%steps_measure
     end if
+    if (learned) call train_proposal()
     call check(esp_get_stats(ctx, acc, rej_w, erun, nmeas), 'esp_get_stats')
     call check(esp_reset_stats(ctx), 'esp_reset_stats')
     call check(esp_run(ctx, 1_c_int, int(inp%steps_mc, c_int), int(inp%steps_measure, c_int), &
@@ -157,6 +160,7 @@ program mc_gpu
       write(out_u, '(1x,a,1000i4)') 'Final configuration = ', smap
       call swap_table('WARMUP2')
     end if
+    if (learned) call train_proposal()
     call check(esp_get_stats(ctx, acc, rej_w, erun, nmeas), 'esp_get_stats')
     call check(esp_reset_stats(ctx), 'esp_reset_stats')
     ! main phase: measurement after every MC step, exchange every pt_steps_swap steps,
@@ -193,6 +197,59 @@ program mc_gpu
   close(out_u)
 
 contains
+
+  ! Learned proposal: records of local conditionals from the warmed-up chains (plain sweeps
+  ! between collections), Adam training on the device, then the measured phase runs with
+  ! mode = ESP_MODE_LEARNED.  Initial weights: He-normal hidden layers, a small output layer.
+  subroutine train_proposal()
+    real(c_float) :: w(ESP_LEARN_NPARAMS)
+    real(c_double) :: losses(1)
+    real(dp) :: u1, u2, g, sd
+    integer :: p, e, nrec, per, got, sz(8), nseed
+    integer, allocatable :: rs(:)
+    integer(c_int64_t) :: s0
+    ! layer sizes: W1 32x5, b1 32, W2 32x32, b2 32, W3 32x32, b3 32, W4 6x32, b4 6
+    sz = [160, 32, 1024, 32, 1024, 32, 192, 6]
+    call random_seed(size=nseed)
+    allocate(rs(nseed))
+    rs = int(seed) + 17 * [(p, p = 1, nseed)]
+    call random_seed(put=rs)
+    p = 0
+    do e = 1, 8
+      sd = 0.0_dp
+      if (e == 1) sd = sqrt(2.0_dp / 5.0_dp)
+      if (e == 3 .or. e == 5) sd = sqrt(2.0_dp / 32.0_dp)
+      if (e == 7) sd = 0.01_dp
+      do k = 1, sz(e)
+        call random_number(u1)
+        call random_number(u2)
+        g = sqrt(-2.0_dp * log(1.0_dp - u1)) * cos(2.0_dp * acos(-1.0_dp) * u2)
+        p = p + 1
+        w(p) = real(sd * g, c_float)
+      end do
+    end do
+    w(ESP_LEARN_NPARAMS - 5:ESP_LEARN_NPARAMS) = [0.0, 0.5, 0.0, -0.5, -0.5, 0.5]
+    call check(esp_learn_set_weights(ctx, w, ESP_LEARN_NPARAMS), 'esp_learn_set_weights')
+    call check(esp_learn_clear(ctx), 'esp_learn_clear')
+    nrec = inp%learned_records
+    per = max(1, nrec / 16)
+    got = 0
+    s0 = seed * 1048576_c_int64_t
+    do while (got < nrec)
+      call check(esp_learn_collect(ctx, int(min(per, nrec - got), c_int), s0 + got), 'esp_learn_collect')
+      got = got + min(per, nrec - got)
+      call check(esp_sweep(ctx, 2_c_int, ESP_MODE_RANDOM, real(inp%tilt, c_double), seed, sweep), 'esp_sweep')
+      sweep = sweep + 2
+    end do
+    write(out_u, '(/1x,a,i10,a,i4,a)') 'Learned proposal: training on ', nrec, ' records, ', &
+      inp%learned_epochs, ' epochs'
+    do e = 1, inp%learned_epochs
+      call check(esp_learn_train(ctx, 1_c_int, 4096_c_int, real(2.0e-3_dp * 0.5_dp**(e - 1), c_float), &
+                 seed + e, losses), 'esp_learn_train')
+      write(out_u, '(1x,a,i4,a,f14.6)') '   epoch ', e, '  mean -log q = ', losses(1)
+    end do
+    mode = ESP_MODE_LEARNED
+  end subroutine
 
   subroutine check(st, what)
     integer(c_int), intent(in) :: st

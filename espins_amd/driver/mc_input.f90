@@ -32,6 +32,9 @@ module mc_input
     real(dp), allocatable :: si_par(:), si_axis(:,:), h_par(:), h_axis(:,:)
     character(len=16) :: tems_mode = 'lin', mcarlo_mode = 'random', initial = 'ferro'
     integer :: tems_num = 1, steps_warmup = 100000, steps_mc = 200000, steps_measure = 2
+    ! mcarlo_mode = learned (an addition of this driver): records and epochs of the training
+    ! between the warm-up (plain 'random' proposals) and the measured phase
+    integer :: learned_records = 1048576, learned_epochs = 4
     real(dp) :: tem_start = 5.0_dp, tem_end = 5.0_dp, tilt = 0.125_dp
     real(dp), allocatable :: tems(:)
     logical :: pt = .false.
@@ -422,8 +425,15 @@ contains
       call input_error('Initial_sconfig not recognised')
     call keyword('mcarlo_mode', found, v)
     if (found) inp%mcarlo_mode = v
-    if (inp%mcarlo_mode(1:5) /= 'const' .and. inp%mcarlo_mode(1:4) /= 'rand') &
+    if (inp%mcarlo_mode(1:5) /= 'const' .and. inp%mcarlo_mode(1:4) /= 'rand' .and. &
+        inp%mcarlo_mode(1:5) /= 'learn') &
       call input_error('mcarlo_mode not recognised')
+    call keyword('learned_records', found, v)
+    if (found) read(v, *) inp%learned_records
+    if (inp%learned_records < 1) call input_error('learned_records must be greater than zero')
+    call keyword('learned_epochs', found, v)
+    if (found) read(v, *) inp%learned_epochs
+    if (inp%learned_epochs < 1) call input_error('learned_epochs must be greater than zero')
     call keyword('tilt_angles_max', found, v)
     if (found) read(v, *) inp%tilt
     call keyword('seeds', found, v)

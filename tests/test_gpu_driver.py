@@ -84,3 +84,19 @@ def test_example02_parallel_tempering(tmp_path):
     perm = [int(x) for x in m[-1].split()]
     assert sorted(perm) == list(range(1, 33))
     assert "swaps" in out
+
+
+def test_example01_learned_proposal(tmp_path):
+    """mcarlo_mode = learned: warm-up with 'random', training on the device, measured phase with
+    the learned proposal; observables as in the published plain run."""
+    _prepare(tmp_path, "ex01_cubic.mcin", "cubic",
+             {"steps_warmup": 2000, "steps_mc": 10000, "seeds": 99, "mcarlo_mode": "learned",
+              "learned_records": 65536, "learned_epochs": 2})
+    out = _run(tmp_path, "cubic")
+    assert "Learned proposal: training on" in out and out.count("mean -log q") == 2
+    adds = [float(x) for x in re.findall(r"adding differences\s*=\s*(\S+)", out)]
+    direct = [float(x) for x in re.findall(r"direct calculations\s*=\s*(\S+)", out)]
+    assert len(adds) == 20 and np.allclose(adds, direct, rtol=1e-6)
+    mine = np.loadtxt(tmp_path / "cubic_mc.dat", comments="#")
+    ref = np.loadtxt(os.path.join(GOLDEN, "examples", "ex01_cubic_mc.dat"), comments="#")
+    assert np.allclose(mine[:, 2], ref[:, 2], rtol=0.02)
