@@ -15,7 +15,7 @@ module espins_hip
             esp_set_supercell, esp_set_model, esp_get_model_info, esp_set_replicas, &
             esp_get_spins, esp_put_spins, esp_energy, esp_run, esp_get_stats, esp_get_map, &
             esp_reset_stats, esp_sync, esp_error_text, esp_sweep, esp_learn_set_weights, &
-            esp_learn_collect, esp_learn_train, esp_learn_clear
+            esp_learn_collect, esp_learn_train, esp_learn_clear, esp_set_trace, esp_get_trace
 
   interface
     integer(c_int) function esp_create(ctx, device) bind(C, name='esp_create')
@@ -122,6 +122,17 @@ module espins_hip
       integer(c_int), value :: nsweeps, mode
       real(c_double), value :: tilt
       integer(c_int64_t), value :: seed, sweep0
+    end function
+    integer(c_int) function esp_set_trace(ctx, capacity) bind(C, name='esp_set_trace')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: capacity
+    end function
+    integer(c_int) function esp_get_trace(ctx, trace, n) bind(C, name='esp_get_trace')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: trace(*)
+      integer(c_int), intent(out) :: n
     end function
     ! learned proposal (no counterpart in the reference)
     integer(c_int) function esp_learn_set_weights(ctx, w, nparams) bind(C, name='esp_learn_set_weights')

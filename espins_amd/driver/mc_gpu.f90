@@ -129,10 +129,12 @@ program mc_gpu
     if (learned) call train_proposal()
     call check(esp_get_stats(ctx, acc, rej_w, erun, nmeas), 'esp_get_stats')
     call check(esp_reset_stats(ctx), 'esp_reset_stats')
+    if (inp%energy_write) call check(esp_set_trace(ctx, int(inp%steps_mc, c_int)), 'esp_set_trace')
     call check(esp_run(ctx, 1_c_int, int(inp%steps_mc, c_int), int(inp%steps_measure, c_int), &
                1_c_int, 0_c_int, 1_c_int, mode, real(inp%tilt, c_double), seed, sweep, 0_c_int64_t, &
                0_c_int), 'esp_run measure')
     steps_mc = inp%steps_mc
+    if (inp%energy_write) call write_energy_series(steps_mc)
     call check(esp_get_stats(ctx, acc, rej, erun, nmeas), 'esp_get_stats')
     call check(esp_energy(ctx, edir), 'esp_energy')
     call check(esp_get_spins(ctx, 0_c_int, spins), 'esp_get_spins')
@@ -163,11 +165,13 @@ program mc_gpu
     if (learned) call train_proposal()
     call check(esp_get_stats(ctx, acc, rej_w, erun, nmeas), 'esp_get_stats')
     call check(esp_reset_stats(ctx), 'esp_reset_stats')
+    if (inp%energy_write) call check(esp_set_trace(ctx, int(steps_mc, c_int)), 'esp_set_trace')
     ! main phase: measurement after every MC step, exchange every pt_steps_swap steps,
     ! parity restarts at 1 (pt.F90:898)
     call check(esp_run(ctx, int(nsw, c_int), int(inp%pt_steps_swap, c_int), int(inp%steps_measure, c_int), &
                1_c_int, 1_c_int, 1_c_int, mode, real(inp%tilt, c_double), seed, sweep, &
                int(nsw_warm, c_int64_t), 1_c_int), 'main')
+    if (inp%energy_write) call write_energy_series(steps_mc)
     call check(esp_get_stats(ctx, acc, rej, erun, nmeas), 'esp_get_stats')
     call check(esp_get_map(ctx, smap, nswap), 'esp_get_map')
     call check(esp_energy(ctx, edir), 'esp_energy')
@@ -197,6 +201,35 @@ program mc_gpu
   close(out_u)
 
 contains
+
+  ! seedname_energy-TNNN.dat, one file per temperature (open_E_unit / write_E / close_E_unit,
+  ! get_quant.F90:1130-1223): header, then the running energy per site of every measured MC
+  ! step, in temperature order.
+  subroutine write_energy_series(nsteps)
+    integer, intent(in) :: nsteps
+    real(c_double), allocatable :: tr(:,:,:)
+    integer(c_int) :: nrec
+    integer :: t, s, u
+    character(len=8) :: d
+    character(len=10) :: tm
+    character(len=300) :: fname
+    allocate(tr(2, R, nsteps))
+    call check(esp_get_trace(ctx, tr, nrec), 'esp_get_trace')
+    call date_and_time(date=d, time=tm)
+    do t = 1, R
+      write(fname, '(a,a,i3.3,a)') trim(seedname), '_energy-T', t, '.dat'
+      open(newunit=u, file=trim(fname), status='replace', action='write')
+      write(u, *) '#written on '//d(7:8)//'/'//d(5:6)//'/'//d(1:4)//' at '//tm(1:2)//':'//tm(3:4)//':'//tm(5:6)
+      write(u, '(3x,a,f9.4)') '# T = ', tems(t)
+      write(u, '(3x,i16)') nsteps
+      do s = 1, min(int(nrec), nsteps)
+        write(u, '(I14,4x,1E16.8)') s, tr(1, t, s) / real(N, dp)
+      end do
+      close(u)
+    end do
+    call check(esp_set_trace(ctx, 0_c_int), 'esp_set_trace')
+    deallocate(tr)
+  end subroutine
 
   ! Learned proposal: records of local conditionals from the warmed-up chains (plain sweeps
   ! between collections), Adam training on the device, then the measured phase runs with

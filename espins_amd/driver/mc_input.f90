@@ -41,6 +41,7 @@ module mc_input
     integer :: pt_steps_swap = 1, pt_print_swap = 10000
     integer :: seed = 0
     logical :: have_seed = .false., open_boundary = .false.
+    logical :: energy_write = .false.     ! _energy-TNNN.dat series (parameters.F90:536-537)
   end type
 
   public :: read_input, input_error
@@ -428,6 +429,8 @@ contains
     if (inp%mcarlo_mode(1:5) /= 'const' .and. inp%mcarlo_mode(1:4) /= 'rand' .and. &
         inp%mcarlo_mode(1:5) /= 'learn') &
       call input_error('mcarlo_mode not recognised')
+    call keyword('energy_write', found, v)
+    if (found) inp%energy_write = is_true(v)
     call keyword('learned_records', found, v)
     if (found) read(v, *) inp%learned_records
     if (inp%learned_records < 1) call input_error('learned_records must be greater than zero')

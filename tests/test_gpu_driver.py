@@ -100,3 +100,21 @@ def test_example01_learned_proposal(tmp_path):
     mine = np.loadtxt(tmp_path / "cubic_mc.dat", comments="#")
     ref = np.loadtxt(os.path.join(GOLDEN, "examples", "ex01_cubic_mc.dat"), comments="#")
     assert np.allclose(mine[:, 2], ref[:, 2], rtol=0.02)
+
+
+def test_energy_series_files(tmp_path):
+    """energy_write = .true.: one seedname_energy-TNNN.dat per temperature with the running
+    energy per site of every measured step; their means are the <E>/N column of _mc.dat."""
+    _prepare(tmp_path, "ex02_cubic.mcin", "cubic",
+             {"steps_warmup": 200, "steps_mc": 500, "supercell_size": "4 4 4", "seeds": 5,
+              "energy_write": ".true."})
+    _run(tmp_path, "cubic")
+    mine = np.loadtxt(tmp_path / "cubic_mc.dat", comments="#")
+    for t in range(mine.shape[0]):
+        lines = (tmp_path / f"cubic_energy-T{t + 1:03d}.dat").read_text().splitlines()
+        assert float(lines[1].split("=")[1]) == pytest.approx(mine[t, 0], abs=1e-4)
+        nsteps = int(lines[2])
+        data = np.array([[float(v) for v in l.split()] for l in lines[3:]])
+        assert data.shape == (nsteps, 2)
+        assert np.array_equal(data[:, 0], np.arange(1, nsteps + 1))
+        assert data[:, 1].mean() == pytest.approx(mine[t, 2], rel=1e-6, abs=1e-7)
