@@ -203,3 +203,98 @@ def test_autocorrelation_time_of_ar1():
         want = (1 + a) / (2 * (1 - a))
         got = Ln.autocorr_time(x)
         assert got == pytest.approx(want, rel=0.08), (a, got, want)
+
+
+# ---------------------------------------------------------------- features -------------------
+def test_features_closed_forms():
+    """The five network inputs in closed-form cases: x0 = log(1 + beta m |G|), x1 = |ghat.a|
+    (1 when G is along the axis, 0 when normal to it), x2 = log T, x3 = beta m^2 (ghat^T Q_B
+    ghat - tr Q_B / 3) (zero for Q_B = 0 and for an isotropic Q_B = b I, (2/3) beta m^2 b for
+    Q_B = b ghat ghat^T), x4 = beta m^2 Delta."""
+    ax = np.array([0.0, 0.0, 1.0])
+    beta, mag, g, b, Delta = 1.6, 1.5, 2.0, 0.35, -0.2
+    x, k0, gh = LO.features(g * ax, np.zeros((3, 3)), Delta, ax, beta, mag)
+    assert k0 == pytest.approx(beta * mag * g, rel=1e-15)
+    assert x[0] == pytest.approx(math.log(1.0 + beta * mag * g), rel=1e-15)
+    assert x[1] == 1.0 and x[3] == 0.0
+    assert x[2] == pytest.approx(math.log(1.0 / beta), rel=1e-15)
+    assert x[4] == pytest.approx(beta * mag * mag * Delta, rel=1e-15)
+    np.testing.assert_allclose(gh, ax, rtol=0, atol=1e-16)
+    # G normal to the axis, B along G
+    G = np.array([0.0, -g, 0.0])
+    x, _, gh = LO.features(G, b * np.outer(G / g, G / g), Delta, ax, beta, mag)
+    assert x[1] == 0.0
+    assert x[3] == pytest.approx(2.0 / 3.0 * beta * mag * mag * b, rel=1e-14)
+    x, _, _ = LO.features(G, b * np.eye(3), Delta, ax, beta, mag)
+    assert x[3] == pytest.approx(0.0, abs=1e-15)
+    # the 45-degree case: |ghat.a| = 1/sqrt(2) for either sign of G
+    for sgn in (1.0, -1.0):
+        x, _, _ = LO.features(sgn * np.array([1.0, 0.0, 1.0]), np.zeros((3, 3)), 0.0, ax, beta, mag)
+        assert x[1] == pytest.approx(1.0 / math.sqrt(2.0), rel=1e-15)
+
+
+def test_record_of_a_ferro_site_in_closed_form():
+    """local_env + features on the ferro state of an SC J1 + B1 + Delta model: every site has
+    G = z J m z-hat and Q_B = z B m^2 z-hat z-hat^T, so x0 = log(1 + beta m z J m),
+    x1 = 1, x3 = (2/3) beta m^2 z B m^2, x4 = beta m^2 Delta (z = 6)."""
+    J, B, Delta, beta, mag = 1.0, 0.1, -0.05, 2.0, 1.0
+    m = Lt.cubic_model("sc", 3, J=(J,), B=B, K=Delta)
+    s = Lt.ferro_spins(m.N, 1, m.mag)[0]
+    for i in (0, 13):
+        rec = LO.record(m, s, 0, i, beta)
+        assert rec[0] == pytest.approx(math.log1p(beta * mag * 6 * J * mag), rel=1e-14)
+        assert rec[1] == 1.0
+        assert rec[3] == pytest.approx(2.0 / 3.0 * beta * mag * mag * 6 * B * mag * mag, rel=1e-14)
+        assert rec[4] == pytest.approx(beta * mag * mag * Delta, rel=1e-15)
+        np.testing.assert_allclose(rec[6:9], [0, 0, 1], atol=1e-16)
+        np.testing.assert_allclose(rec[12:15], [0, 0, 1], atol=1e-16)
+
+
+# ---------------------------------------------------------------- sweep network --------------
+def test_bf16_round_matches_torch_bfloat16():
+    """Round to nearest even BF16, against torch's own float -> bfloat16 conversion, including
+    exact ties (low 16 bits 0x8000) on both parities."""
+    import torch
+    rng = np.random.default_rng(21)
+    a = np.concatenate([rng.normal(0, 3, 4000), rng.uniform(-1e-3, 1e-3, 1000)]).astype(np.float32)
+    u = a.view(np.uint32)
+    ties = ((u[:200] & 0xFFFF0000) | 0x8000).astype(np.uint32).view(np.float32)
+    a = np.concatenate([a, ties])
+    want = torch.from_numpy(a).to(torch.bfloat16).to(torch.float64).numpy()
+    np.testing.assert_array_equal(LO.bf16_round(a), want)
+
+
+def test_forward_sweep_net_hand_computed_case():
+    """One path through the network with hand-chosen weights: x0 = 1 feeds hidden unit 0 of
+    layer 1 with weight 1; layers 2 and 3 copy unit 0 with weight 1; output 0 reads unit 0 with
+    weight 1 and output 1 is the bias 0.1.
+      a1 = bf16(gelu(1))            = bf16(0.84134474...) = 0.83984375 (= 215/256)
+      a2 = bf16(gelu(0.83984375))   = bf16(0.67080...)    = 0.671875   (= 172/256)
+      a3 = gelu(0.671875)           (full precision)
+      o0 = a3, o1 = bf16(0.1) = 0.10009765625, others 0
+    gelu(x) = x Phi(x) (exact erf)."""
+    p = {k: np.zeros(s) for k, s in LO._SHAPES}
+    p["W1"][0, 0] = 1.0
+    p["W2"][0, 0] = 1.0
+    p["W3"][0, 0] = 1.0
+    p["W4"][0, 0] = 1.0
+    p["b4"][1] = 0.1
+    w = LO.pack(p)
+    X = np.array([[1.0, 0.0, 0.0, 0.0, 0.0]])
+    phi = lambda t: 0.5 * (1.0 + math.erf(t / math.sqrt(2.0)))
+    a1 = 215.0 / 256.0
+    assert abs(1.0 * phi(1.0) - a1) < 2.0 ** -9        # within half a BF16 ulp at this magnitude
+    a2 = 172.0 / 256.0
+    assert abs(a1 * phi(a1) - a2) < 2.0 ** -9
+    a3 = a2 * phi(a2)
+    o = LO.forward_sweep_net(w, X)[0]
+    assert o[0] == pytest.approx(a3, rel=1e-14)
+    assert o[1] == 0.10009765625
+    assert np.all(o[2:] == 0.0)
+    # the full-precision network keeps the unrounded chain and differs from it
+    of, _ = LO.forward(w, X)
+    want = (phi(1.0) * 1.0)
+    want = want * phi(want)
+    want = want * phi(want)
+    assert of[0, 0] == pytest.approx(want, rel=1e-14)
+    assert abs(of[0, 0] - o[0]) > 1e-4

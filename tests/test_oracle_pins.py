@@ -367,3 +367,75 @@ def test_binder_cumulant_and_estimators_known_values():
     out = O.finalize(acc, 200000, T, N)
     assert abs(out[4]) < 0.02
     assert out[2] == pytest.approx(4.0 / T ** 2 / N, rel=0.02)
+
+
+# ---------------------------------------------------------------- magnetisation --------------
+@pytest.mark.parametrize("mag", [1.0, 2.5])
+def test_magnetization2_closed_forms(mag):
+    """magnetization_get (get_quant.F90:340-363) returns |sum_j S_j / N|^2: ferro gives m^2, a
+    two-sublattice Neel state 0, and a cone of half-angle alpha with evenly spaced azimuths
+    (m cos alpha)^2.  A dropped 1/N, a sum of squares or a missing component fails one."""
+    L = 6
+    m = Lt.cubic_model("sc", L, J=(1.0,), builder=O.build_table_fn)
+    m.mag = np.full(m.N, mag)
+    o = O.Oracle(m)
+    s = Lt.ferro_spins(m.N, 1, m.mag)[0]
+    assert o.magnetization2(s) == pytest.approx(mag * mag, rel=1e-14)
+    idx = np.arange(m.N)
+    x, y, z = idx % L, (idx // L) % L, idx // (L * L)
+    s[:, 2] = np.where((x + y + z) % 2 == 0, mag, -mag)
+    assert o.magnetization2(s) == pytest.approx(0.0, abs=1e-28)
+    alpha = 0.7
+    phi = 2 * math.pi * idx / m.N
+    cone = mag * np.stack([math.sin(alpha) * np.cos(phi), math.sin(alpha) * np.sin(phi),
+                           np.full(m.N, math.cos(alpha))], 1)
+    assert o.magnetization2(cone) == pytest.approx((mag * math.cos(alpha)) ** 2, rel=1e-12)
+    # a uniform tilt of the whole ferro state keeps |m|^2
+    tilt = np.tile(mag * np.array([0.6, 0.0, 0.8]), (m.N, 1))
+    assert o.magnetization2(tilt) == pytest.approx(mag * mag, rel=1e-14)
+
+
+# ---------------------------------------------------------------- colouring check -------------
+def _checkerboard(L, N):
+    idx = np.arange(N)
+    return ((idx % L + (idx // L) % L + idx // (L * L)) % 2).astype(np.int32)
+
+
+@pytest.mark.parametrize("term", ["J", "B", "D"])
+def test_colouring_check_rejects_improper_colourings(term):
+    """orc_check_colouring must accept a proper colouring and reject one that gives two bonded
+    sites the same colour, for bonds that come from the J, the B or the D table alone.  The
+    checkerboard of SC is proper for nearest neighbours and improper for the second shell."""
+    L = 4
+    base = Lt.cubic_model("sc", L, J=(1.0,), builder=O.build_table_fn)
+    col = _checkerboard(L, base.N)
+    assert O.Oracle(base).check_colouring(col)
+    # one site moved to its neighbour's colour
+    bad = col.copy()
+    bad[0] = col[int(base.J.nbr[0, 0]) - 1]
+    assert not O.Oracle(base).check_colouring(bad)
+    # second-shell bonds present only in one table: the checkerboard is no longer proper
+    nb, bf, bm = base.nb, base.basis_frac, np.ones(1)
+    m = Lt.cubic_model("sc", L, J=(1.0,), builder=O.build_table_fn)
+    if term == "J":
+        m = Lt.cubic_model("sc", L, J=(1.0, 0.3), builder=O.build_table_fn)
+    elif term == "B":
+        m.B = Lt.build_table(nb, bf, bm, m.L, Lt.shell_rows("sc", [0.0, 0.2]), 1, builder=O.build_table_fn)
+    else:
+        m.D = Lt.build_table(nb, bf, bm, m.L, Lt.shell_rows("sc", [], dm=0.3, dm_shell=2), 3,
+                             builder=O.build_table_fn)
+    assert not O.Oracle(m).check_colouring(col)
+    # a colouring of the sublattices modulo 2 in every axis (8 colours) separates shell 2
+    idx = np.arange(m.N)
+    col8 = ((idx % L) % 2 + 2 * (((idx // L) % L) % 2) + 4 * ((idx // (L * L)) % 2)).astype(np.int32)
+    assert O.Oracle(m).check_colouring(col8)
+
+
+def test_colouring_check_ignores_self_bonds():
+    """A supercell one cell thick along an axis bonds a site to itself (the bond wraps onto
+    the same site); such bonds involve only the updated site and do not constrain colours."""
+    rows = Lt.shell_rows("sc", [1.0])
+    t = Lt.build_table(1, np.zeros((1, 3)), np.ones(1), (1, 1, 2), rows, 1, builder=O.build_table_fn)
+    m = Lt.Model("sc", (1, 1, 2), 1, np.zeros((1, 3)), np.ones(2), t)
+    assert O.Oracle(m).check_colouring(np.array([0, 1], np.int32))
+    assert not O.Oracle(m).check_colouring(np.array([0, 0], np.int32))
