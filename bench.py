@@ -168,7 +168,11 @@ def run_batch(args):
     log(f"batch: {len(chains)} chains, {total} spin updates per sweep, set up in "
         f"{time.perf_counter() - t0:.1f} s")
     t1 = time.perf_counter()
-    b.autotune(nsweeps=2, cache=args.tune_cache)
+    if args.batch_path == "auto":
+        b.autotune(nsweeps=2, cache=args.tune_cache)
+    else:                     # every chain on one kernel path, one lane per site
+        for k in range(len(chains)):
+            b.set_chain_config(k, 1 if args.batch_path == "global" else 0, 1)
     shapes = {}
     for k in range(len(chains)):
         i = b.info(k)
@@ -353,6 +357,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chains", type=int, default=256, help="batch_c3: number of chains")
     ap.add_argument("--tune-cache", default=None, help="batch_c3: autotuner cache file")
+    ap.add_argument("--batch-path", default="auto", choices=("auto", "global", "fit"),
+                    help="batch_c3: autotuned kernel shapes, every chain on the global-memory "
+                         "path, or LDS wherever a replica fits")
     ap.add_argument("--records", type=int, default=1 << 20, help="learned_fcc32: training records")
     ap.add_argument("--epochs", type=int, default=4, help="learned_fcc32: training epochs")
     ap.add_argument("--tilt", type=float, default=0.125, help="learned_fcc32: 'const' cone tilt")

@@ -56,6 +56,7 @@ _SIGS = {
     "esp_set_path": (_i, [_p, _i]),
     "esp_set_stream_base": (_i, [_p, C.c_uint32]),
     "esp_set_tps": (_i, [_p, _i]),
+    "esp_set_indexing": (_i, [_p, _i]),
     "esp_plan_model": (_i, [_i, _i, _dp, _i, _ip, _ip, _dp, _i, _ip, _ip, _dp, _i, _ip, _ip, _dp,
                             _ip, _ip, _ip, _ip]),
     "esp_get_coloring": (_i, [_p, _ip, _ip]),

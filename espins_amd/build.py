@@ -40,15 +40,19 @@ def build_hip(verbose: bool = False) -> str:
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     objdir = os.path.join(PKG, "_obj")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    jobs = []
     for src in ("esp_model.cpp", "esp_hip.hip", "esp_batch.hip", "esp_learn.hip"):
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
         obj = os.path.join(objdir, src + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        _run([hipcc, *HIP_FLAGS, *lang, "-c", path, "-o", obj])
-        objs.append(obj)
+        jobs.append(([hipcc, *HIP_FLAGS, *lang, "-c", path, "-o", obj], obj))
+    # the translation units are independent: compile them side by side
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        list(ex.map(lambda j: _run(j[0]), jobs))
+    objs = [o for _, o in jobs]
     tmp = LIB + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)

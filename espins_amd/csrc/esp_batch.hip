@@ -27,7 +27,7 @@ namespace {
 
 // ---- batched kernels -------------------------------------------------------------------
 // tiles: (chain, replica, block, 0); chain < 0 = padding tile (XCD interleave)
-template <int MODE, bool CMP, bool HB, bool HD, bool SELF>
+template <int MODE, int LAY, bool HB, bool HD, bool SELF>
 __global__ __launch_bounds__(kBlock) void k_bphase(const KArgs *chains, const int4 *tiles, int colour,
                                                    int sweep_off) {
   extern __shared__ double s_ctab[];
@@ -38,7 +38,7 @@ __global__ __launch_bounds__(kBlock) void k_bphase(const KArgs *chains, const in
   a.c_begin = a.col_off[colour];
   a.c_count = a.col_off[colour + 1] - a.c_begin;
   a.sweep_off = sweep_off;
-  phase_body<MODE, CMP, HB, HD, SELF>(a, t.y, t.z, s_ctab);
+  phase_body<MODE, LAY, HB, HD, SELF>(a, t.y, t.z, s_ctab);
 }
 
 // LDS-resident chains: one block per (chain, replica)
@@ -91,14 +91,19 @@ __global__ __launch_bounds__(1024) void k_bfinal(const FinArgs *fa, Clock *clock
 using BPhaseFn = void (*)(const KArgs *, const int4 *, int, int);
 using BLdsFn = void (*)(const KArgs *, const LdsMeas *, const int2 *, int, int);
 
-template <int MODE, bool CMP, bool HB, bool HD>
-BPhaseFn bp4(bool self) { return self ? k_bphase<MODE, CMP, HB, HD, true> : k_bphase<MODE, CMP, HB, HD, false>; }
-template <int MODE, bool CMP, bool HB>
-BPhaseFn bp3(bool hd, bool self) { return hd ? bp4<MODE, CMP, HB, true>(self) : bp4<MODE, CMP, HB, false>(self); }
-template <int MODE, bool CMP>
-BPhaseFn bp2(bool hb, bool hd, bool self) { return hb ? bp3<MODE, CMP, true>(hd, self) : bp3<MODE, CMP, false>(hd, self); }
+// variant bits: 0-1 coefficient layout (LAY_*), 2 biquadratic, 3 DM, 4 self bonds
+template <int MODE, int LAY, bool HB, bool HD>
+BPhaseFn bp4(bool self) { return self ? k_bphase<MODE, LAY, HB, HD, true> : k_bphase<MODE, LAY, HB, HD, false>; }
+template <int MODE, int LAY, bool HB>
+BPhaseFn bp3(bool hd, bool self) { return hd ? bp4<MODE, LAY, HB, true>(self) : bp4<MODE, LAY, HB, false>(self); }
+template <int MODE, int LAY>
+BPhaseFn bp2(bool hb, bool hd, bool self) { return hb ? bp3<MODE, LAY, true>(hd, self) : bp3<MODE, LAY, false>(hd, self); }
 template <int MODE>
-BPhaseFn bp1(int v) { return (v & 1) ? bp2<MODE, true>(v & 2, v & 4, v & 8) : bp2<MODE, false>(v & 2, v & 4, v & 8); }
+BPhaseFn bp1(int v) {
+  const int lay = v & 3;
+  if (lay == LAY_IMP) return bp2<MODE, LAY_IMP>(v & 4, v & 8, v & 16);
+  return lay == LAY_CMP ? bp2<MODE, LAY_CMP>(v & 4, v & 8, v & 16) : bp2<MODE, LAY_BOND>(v & 4, v & 8, v & 16);
+}
 BPhaseFn bphase_fn(int mode, int variant) { return mode == M_RANDOM ? bp1<M_RANDOM>(variant) : bp1<M_CONST>(variant); }
 
 template <int MODE, bool CMP, bool HB, bool HD>
@@ -108,11 +113,12 @@ BLdsFn bl3(bool hd, bool self) { return hd ? bl4<MODE, CMP, HB, true>(self) : bl
 template <int MODE, bool CMP>
 BLdsFn bl2(bool hb, bool hd, bool self) { return hb ? bl3<MODE, CMP, true>(hd, self) : bl3<MODE, CMP, false>(hd, self); }
 template <int MODE>
-BLdsFn bl1(int v) { return (v & 1) ? bl2<MODE, true>(v & 2, v & 4, v & 8) : bl2<MODE, false>(v & 2, v & 4, v & 8); }
+BLdsFn bl1(int v) { return (v & 3) ? bl2<MODE, true>(v & 4, v & 8, v & 16) : bl2<MODE, false>(v & 4, v & 8, v & 16); }
 BLdsFn blds_fn(int mode, int variant) { return mode == M_RANDOM ? bl1<M_RANDOM>(variant) : bl1<M_CONST>(variant); }
 
 int variant_of(const esp::HostModel &m) {
-  return (m.compressed ? 1 : 0) | (m.has_B ? 2 : 0) | (m.has_D ? 4 : 0) | (m.has_self ? 8 : 0);
+  const int lay = m.compressed ? (m.implicit ? LAY_IMP : LAY_CMP) : LAY_BOND;
+  return lay | (m.has_B ? 4 : 0) | (m.has_D ? 8 : 0) | (m.has_self ? 16 : 0);
 }
 
 // energy of one chain's replicas (single-context kernels reused)
@@ -121,7 +127,7 @@ void energy_launch(const KArgs &a, int nblk, double *partial, size_t shm, hipStr
   hipLaunchKernelGGL((k_energy<CMP, HB, HD>), dim3(nblk, a.R), dim3(kBlock), shm, s, a, nblk, partial);
 }
 void launch_energy(const KArgs &a, int variant, int nblk, double *partial, size_t shm, hipStream_t s) {
-  const bool c = variant & 1, b = variant & 2, d = variant & 4;
+  const bool c = (variant & 3) != 0, b = variant & 4, d = variant & 8;
   if (c) {
     if (b) d ? energy_launch<true, true, true>(a, nblk, partial, shm, s) : energy_launch<true, true, false>(a, nblk, partial, shm, s);
     else d ? energy_launch<true, false, true>(a, nblk, partial, shm, s) : energy_launch<true, false, false>(a, nblk, partial, shm, s);
@@ -148,6 +154,8 @@ struct Chain {
   unsigned long long *rej = nullptr, *nswap = nullptr, *nmeas = nullptr;
   double *mpart = nullptr;   // per-wave spin sums of the colour phases (global-path chains)
   int mpart_nw = 0;
+  int *grp = nullptr, *gslot = nullptr;   // implicit geometry (esp_geo.h)
+  double *magb = nullptr;
 };
 
 }  // namespace
@@ -232,6 +240,7 @@ void free_chain(Chain &c) {
   bfree(c.slot_rep); bfree(c.rep_slot); bfree(c.basis); bfree(c.rej); bfree(c.nswap); bfree(c.nmeas);
   bfree(c.mpart);
   c.mpart_nw = 0;
+  bfree(c.grp); bfree(c.gslot); bfree(c.magb);
 }
 
 void drop_plan(esp_batch *b) {
@@ -255,6 +264,9 @@ KArgs chain_args(const Chain &c, int k) {
   a.N = c.hm.N; a.Ns = c.Ns; a.nb = c.hm.nb; a.zmax = c.hm.zmax; a.R = c.R; a.ncol = c.hm.ncol;
   a.has_si = c.hm.has_si; a.has_h = c.hm.has_h; a.tps = c.tps;
   a.stream_base = (uint32_t)k << 16;
+  a.geo = c.hm.geo;
+  a.grp = c.grp; a.gslot = c.gslot; a.magb = c.magb;
+  a.rows_max = c.hm.rows_max;
   return a;
 }
 
@@ -357,7 +369,7 @@ int build_plan(esp_batch *b, double tilt, uint64_t seed) {
     for (int k = 0; k < nc; ++k) {
       const Chain &c = b->chains[k];
       if (c.variant != v.variant) continue;
-      shm_p = std::max(shm_p, c.ctab_bytes);
+      shm_p = std::max(shm_p, phase_lds_bytes(c.variant & 3, c.hm.nb, c.hm.zmax, c.hm.rows_max));
       if (!c.lds) continue;
       shm_l = std::max(shm_l, c.lds_bytes);
       for (int r = 0; r < c.R; ++r) li.push_back(make_int2(k, r));
@@ -525,8 +537,11 @@ int esp_batch_add_chain(esp_batch *b, int N, int nb, const double *mag, int zJ, 
           c.hm.coef[((size_t)l * N + d) * esp::kCoef + q] =
               c.hm.ctab[((size_t)c.hm.basis[d] * c.hm.zmax + l) * esp::kCoef + q];
     c.hm.ctab.clear();
+    c.hm.implicit = false;
   }
   if (!c.hm.compressed) c.ctab_bytes = 0;
+  if (c.hm.implicit && phase_lds_bytes(LAY_IMP, c.hm.nb, c.hm.zmax, c.hm.rows_max) > 49152)
+    c.hm.implicit = false;
   if ((size_t)c.hm.zmax * N * sizeof(int) >= (1ull << 32)) return bfail(b, ESP_EINVAL, "bond table beyond 4 GiB");
   if (L3) std::memcpy(c.L, L3, sizeof c.L);
   c.R = R;
@@ -576,6 +591,11 @@ int esp_batch_commit(esp_batch *b) {
     if ((rc = bupload(b, c.basis, m.basis.data(), m.basis.size()))) return rc;
     if ((rc = bupload(b, c.si4, m.si4.data(), m.si4.size()))) return rc;
     if ((rc = bupload(b, c.h4, m.h4.data(), m.h4.size()))) return rc;
+    if (m.implicit) {
+      if ((rc = bupload(b, c.grp, m.grp.data(), m.grp.size()))) return rc;
+      if ((rc = bupload(b, c.gslot, m.gslot.data(), m.gslot.size()))) return rc;
+      if ((rc = bupload(b, c.magb, m.magb.data(), m.magb.size()))) return rc;
+    }
     if (m.compressed) {
       if ((rc = bupload(b, c.ctab, m.ctab.data(), m.ctab.size()))) return rc;
     } else {
@@ -738,8 +758,9 @@ int time_chain(esp_batch *b, int k, bool lds, int tps, int nsweeps, int mode, do
   } else {
     for (int s = 0; s < nsweeps; ++s)
       for (int col = 0; col < c.hm.ncol; ++col)
-        hipLaunchKernelGGL(bphase_fn(mode, c.variant), dim3(nt[col]), dim3(kBlock), c.ctab_bytes,
-                           b->stream, (const KArgs *)d_one, (const int4 *)tiles[col], col, s);
+        hipLaunchKernelGGL(bphase_fn(mode, c.variant), dim3(nt[col]), dim3(kBlock),
+                           phase_lds_bytes(c.variant & 3, c.hm.nb, c.hm.zmax, c.hm.rows_max), b->stream,
+                           (const KArgs *)d_one, (const int4 *)tiles[col], col, s);
   }
   BCHK(b, hipGetLastError());
   BCHK(b, hipEventRecord(e1, b->stream));

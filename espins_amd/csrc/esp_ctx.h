@@ -21,6 +21,7 @@ struct esp_ctx {
   int Lhint[3] = {0, 0, 0};       // supercell size for the periodic colouring
   int path = 0;                   // 0 auto, 1 global-memory phases, 2 LDS-resident
   uint32_t stream_base = 0;       // Philox stream of replica r: stream_base + r
+  int indexing = 0;               // 0 implicit geometry when the lattice allows, 1 explicit tables
   bool lds_fits = false;
   // model
   esp::HostModel hm;
@@ -29,6 +30,8 @@ struct esp_ctx {
   double *d_cJ = nullptr, *d_cB = nullptr, *d_cD = nullptr, *d_ctab = nullptr;
   double *d_mag = nullptr, *d_si4 = nullptr, *d_h4 = nullptr;
   uint8_t *d_basis = nullptr;
+  int *d_grp = nullptr, *d_gslot = nullptr;   // implicit geometry (esp_geo.h)
+  double *d_magb = nullptr;
   bool use_lds = false;
   size_t lds_bytes = 0, ctab_bytes = 0;
   // replicas
@@ -137,6 +140,7 @@ void free_model(esp_ctx *c) {
   dfree(c->d_nbr); dfree(c->d_dev2ref); dfree(c->d_ref2dev); dfree(c->d_col_off);
   dfree(c->d_cJ); dfree(c->d_cB); dfree(c->d_cD); dfree(c->d_ctab); dfree(c->d_mag);
   dfree(c->d_si4); dfree(c->d_h4); dfree(c->d_basis);
+  dfree(c->d_grp); dfree(c->d_gslot); dfree(c->d_magb);
   c->have_model = false;
   drop_graph(c);
 }
@@ -155,6 +159,9 @@ KArgs base_args(esp_ctx *c) {
   a.N = m.N; a.Ns = c->Ns; a.nb = m.nb; a.zmax = m.zmax; a.R = c->R; a.ncol = m.ncol;
   a.has_si = m.has_si; a.has_h = m.has_h;
   a.stream_base = c->stream_base;
+  a.geo = m.geo;
+  a.grp = c->d_grp; a.gslot = c->d_gslot; a.magb = c->d_magb;
+  a.rows_max = m.rows_max;
   return a;
 }
 

@@ -39,35 +39,49 @@ int auto_tps(const esp_ctx *c) {
   return z <= 8 ? 1 : (z <= 20 ? 2 : 4);
 }
 
-// kernel tables: [mode][cmp][hb][hd][self]
+// kernel tables: [mode][layout][hb][hd][self]
 using PhaseFn = void (*)(KArgs);
 using LdsFn = void (*)(KArgs, int, LdsMeas);
 using EnergyFn = void (*)(KArgs, int, double *);
 
-template <int MODE, bool CMP, bool HB, bool HD, int CH>
-PhaseFn ph4(bool self) { return self ? k_phase<MODE, CMP, HB, HD, true, CH> : k_phase<MODE, CMP, HB, HD, false, CH>; }
-template <int MODE, bool CMP, bool HB, int CH>
-PhaseFn ph3(bool hd, bool self) { return hd ? ph4<MODE, CMP, HB, true, CH>(self) : ph4<MODE, CMP, HB, false, CH>(self); }
-template <int MODE, bool CMP, int CH>
-PhaseFn ph2(bool hb, bool hd, bool self) { return hb ? ph3<MODE, CMP, true, CH>(hd, self) : ph3<MODE, CMP, false, CH>(hd, self); }
+template <int MODE, int LAY, bool HB, bool HD, int CH>
+PhaseFn ph4(bool self) { return self ? k_phase<MODE, LAY, HB, HD, true, CH> : k_phase<MODE, LAY, HB, HD, false, CH>; }
+template <int MODE, int LAY, bool HB, int CH>
+PhaseFn ph3(bool hd, bool self) { return hd ? ph4<MODE, LAY, HB, true, CH>(self) : ph4<MODE, LAY, HB, false, CH>(self); }
+template <int MODE, int LAY, int CH>
+PhaseFn ph2(bool hb, bool hd, bool self) { return hb ? ph3<MODE, LAY, true, CH>(hd, self) : ph3<MODE, LAY, false, CH>(hd, self); }
 template <int MODE, int CH>
-PhaseFn ph1(bool cmp, bool hb, bool hd, bool self) { return cmp ? ph2<MODE, true, CH>(hb, hd, self) : ph2<MODE, false, CH>(hb, hd, self); }
+PhaseFn ph1(int lay, bool hb, bool hd, bool self) {
+  if (lay == LAY_IMP) return ph2<MODE, LAY_IMP, CH>(hb, hd, self);
+  return lay == LAY_CMP ? ph2<MODE, LAY_CMP, CH>(hb, hd, self) : ph2<MODE, LAY_BOND, CH>(hb, hd, self);
+}
 template <int CH>
-PhaseFn phase_fn_ch(int mode, const esp::HostModel &m) {
-  if (mode == M_RANDOM) return ph1<M_RANDOM, CH>(m.compressed, m.has_B, m.has_D, m.has_self);
-  if (mode == M_CONST) return ph1<M_CONST, CH>(m.compressed, m.has_B, m.has_D, m.has_self);
-  return ph1<M_HOSTFED, CH>(m.compressed, m.has_B, m.has_D, m.has_self);
+PhaseFn phase_fn_ch(int mode, int lay, const esp::HostModel &m) {
+  if (mode == M_RANDOM) return ph1<M_RANDOM, CH>(lay, m.has_B, m.has_D, m.has_self);
+  if (mode == M_CONST) return ph1<M_CONST, CH>(lay, m.has_B, m.has_D, m.has_self);
+  return ph1<M_HOSTFED, CH>(lay, m.has_B, m.has_D, m.has_self);
 }
 // Lanes of one colour phase of every replica (mean over colours).
 int64_t launch_lanes(const esp_ctx *c) {
   return (int64_t)c->hm.N * c->tps / std::max(1, c->hm.ncol) * c->R;
 }
 
+// Coefficient layout of the colour phases of this context.
+int phase_layout(const esp_ctx *c) {
+  const esp::HostModel &m = c->hm;
+  if (m.implicit && m.compressed && c->indexing == 0) return LAY_IMP;
+  return m.compressed ? LAY_CMP : LAY_BOND;
+}
+size_t phase_shm(const esp_ctx *c) {
+  return phase_lds_bytes(phase_layout(c), c->hm.nb, c->hm.zmax, c->hm.rows_max);
+}
+
 // Wide chunks (more gathers in flight per lane, more registers) when one colour phase of all
 // replicas has too few waves to hide memory latency by occupancy: < 4 waves per SIMD.
-PhaseFn phase_fn(int mode, const esp::HostModel &m, int64_t lanes_per_launch) {
+PhaseFn phase_fn(int mode, const esp_ctx *c, int64_t lanes_per_launch) {
   const bool wide = lanes_per_launch < 4 * 1024 * 64;
-  return wide ? phase_fn_ch<kChunkWide>(mode, m) : phase_fn_ch<kChunk>(mode, m);
+  const int lay = phase_layout(c);
+  return wide ? phase_fn_ch<kChunkWide>(mode, lay, c->hm) : phase_fn_ch<kChunk>(mode, lay, c->hm);
 }
 template <int MODE, bool CMP, bool HB, bool HD>
 LdsFn ld4(bool self) { return self ? k_lds<MODE, CMP, HB, HD, true> : k_lds<MODE, CMP, HB, HD, false>; }
@@ -147,8 +161,8 @@ int enqueue_phase_sweeps(esp_ctx *c, int mode, double tilt, uint64_t seed, int n
   }
   const esp::HostModel &m = c->hm;
   KArgs a = phase_args(c, tilt, seed);
-  PhaseFn fn = phase_fn(mode, m, launch_lanes(c));
-  const size_t shm = m.compressed ? c->ctab_bytes : 0;
+  PhaseFn fn = phase_fn(mode, c, launch_lanes(c));
+  const size_t shm = phase_shm(c);
   for (int s = 0; s < nsweeps; ++s)
     for (int col = 0; col < m.ncol; ++col) {
       set_colour(a, m, col);
@@ -372,7 +386,9 @@ int esp_set_model(esp_ctx *c, int N, int nb, const double *mag, int zJ, const in
           hm.coef[((size_t)l * N + d) * esp::kCoef + k] = hm.ctab[((size_t)a * hm.zmax + l) * esp::kCoef + k];
     }
     hm.ctab.clear();
+    hm.implicit = false;
   }
+  if (hm.implicit && phase_lds_bytes(LAY_IMP, hm.nb, hm.zmax, hm.rows_max) > 49152) hm.implicit = false;
   if ((size_t)hm.zmax * N * sizeof(int) >= (1ull << 32) || (size_t)(N + 64) * 8 >= (1ull << 32))
     return fail(c, ESP_EINVAL, "bond table or replica beyond 4 GiB (32-bit device offsets)");
   free_replicas(c);
@@ -388,6 +404,11 @@ int esp_set_model(esp_ctx *c, int N, int nb, const double *mag, int zJ, const in
   if ((rc = upload(c, c->d_basis, m.basis))) return rc;
   if ((rc = upload(c, c->d_si4, m.si4))) return rc;
   if ((rc = upload(c, c->d_h4, m.h4))) return rc;
+  if (m.implicit) {
+    if ((rc = upload(c, c->d_grp, m.grp))) return rc;
+    if ((rc = upload(c, c->d_gslot, m.gslot))) return rc;
+    if ((rc = upload(c, c->d_magb, m.magb))) return rc;
+  }
   if (m.compressed) {
     if ((rc = upload(c, c->d_ctab, m.ctab))) return rc;
   } else {
@@ -440,7 +461,11 @@ int esp_plan_model(int N, int nb, const double *mag, int zJ, const int *j_cnt, c
   *ncol = hm.ncol;
   std::memcpy(colour, hm.colour.data(), sizeof(int) * N);
   if (info) {
-    info[0] = hm.N; info[1] = hm.zmax; info[2] = hm.ncol; info[3] = hm.compressed ? 1 : 0;
+    const bool imp = hm.implicit && hm.compressed &&
+                     (size_t)hm.nb * hm.zmax * 5 * sizeof(double) <= 32768 &&
+                     phase_lds_bytes(LAY_IMP, hm.nb, hm.zmax, hm.rows_max) <= 49152;
+    info[0] = hm.N; info[1] = hm.zmax; info[2] = hm.ncol;
+    info[3] = hm.compressed ? (imp ? LAY_IMP : LAY_CMP) : LAY_BOND;
     info[4] = 0;
   }
   return ESP_OK;
@@ -465,6 +490,13 @@ int esp_set_tps(esp_ctx *c, int tps) {
     c->tps = auto_tps(c);
     drop_graph(c);
   }
+  return ESP_OK;
+}
+
+int esp_set_indexing(esp_ctx *c, int mode) {
+  if (!c || (mode != 0 && mode != 1)) return ESP_EINVAL;
+  c->indexing = mode;
+  drop_graph(c);
   return ESP_OK;
 }
 
@@ -495,7 +527,7 @@ int esp_get_model_info(esp_ctx *c, int *out) {
   if (!c || !out) return ESP_EINVAL;
   if (!c->have_model) return fail(c, ESP_ESTATE, "no model");
   out[0] = c->hm.N; out[1] = c->hm.zmax; out[2] = c->hm.ncol;
-  out[3] = c->hm.compressed ? 1 : 0; out[4] = c->use_lds ? 1 : 0;
+  out[3] = phase_layout(c); out[4] = c->use_lds ? 1 : 0;
   out[5] = c->tps;
   return ESP_OK;
 }
@@ -619,7 +651,7 @@ int esp_sweep_hostfed(esp_ctx *c, const double *prop, const double *unif, uint8_
   KArgs a = base_args(c);   // one lane per site here: no per-wave sums
   a.prop = c->d_prop; a.unif = c->d_unif; a.accepted = c->d_accd;
   c->mpart_valid = false;
-  PhaseFn fn = phase_fn(M_HOSTFED, m, launch_lanes(c));
+  PhaseFn fn = phase_fn(M_HOSTFED, c, launch_lanes(c));
   if ((rc = set_clock(c, 0, 0, 1))) return rc;
   for (int col = 0; col < m.ncol; ++col) {
     a.c_begin = m.col_off[col];
@@ -628,8 +660,7 @@ int esp_sweep_hostfed(esp_ctx *c, const double *prop, const double *unif, uint8_
     a.tps = 1;
     a.nblk = (a.c_count + kBlock - 1) / kBlock;
     a.xcd_remap = (c->R % 8 == 0);
-    hipLaunchKernelGGL(fn, dim3(a.nblk * c->R), dim3(kBlock), m.compressed ? c->ctab_bytes : 0,
-                       c->stream, a);
+    hipLaunchKernelGGL(fn, dim3(a.nblk * c->R), dim3(kBlock), phase_shm(c), c->stream, a);
   }
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(accepted, c->d_accd, n, hipMemcpyDeviceToHost, c->stream));
@@ -732,15 +763,15 @@ int esp_time_sweeps(esp_ctx *c, int nsweeps, int mode, double tilt, uint64_t see
     k = 1;
   } else {
     KArgs a = phase_args(c, tilt, seed);
-    PhaseFn fn = phase_fn(mode, m, launch_lanes(c));
+    PhaseFn fn = phase_fn(mode, c, launch_lanes(c));
+    const size_t shm = phase_shm(c);
     for (int sw = 0; sw < nsweeps; ++sw)
       for (int col = 0; col < m.ncol; ++col) {
         set_colour(a, m, col);
         if (a.c_count == 0) continue;
         a.sweep_off = sw;
         // start/stop stamped by the dispatch packet itself
-        hipExtLaunchKernelGGL(fn, dim3(a.nblk * c->R), dim3(kBlock),
-                              m.compressed ? c->ctab_bytes : 0, c->stream, ev[2 * k],
+        hipExtLaunchKernelGGL(fn, dim3(a.nblk * c->R), dim3(kBlock), shm, c->stream, ev[2 * k],
                               ev[2 * k + 1], 0, a);
         ++k;
       }

@@ -11,6 +11,7 @@
 // in LDS when the lattice is periodic, else cJ/cB[zmax][N], cD[zmax][N][3].
 #pragma once
 #include "esp_device.h"
+#include "esp_geo.h"
 
 namespace espk {
 struct Clock {                   // device-side schedule counters (graph-replayable)
@@ -58,6 +59,12 @@ struct KArgs {
   // phase `col` overwrites its waves' entries
   double *mpart;
   int col, mpart_nw;
+  // implicit geometry (esp_geo.h), used by the global-memory colour phases when set
+  espg::Geo geo;
+  const int *grp;                // [ngroups]
+  const int *gslot;              // [ngroups][zmax][2]
+  const double *magb;            // [nb]
+  int rows_max;                  // group rows one block stages in LDS
 };
 
 enum { M_RANDOM = 0, M_CONST = 1, M_HOSTFED = 2 };
@@ -87,13 +94,40 @@ struct Proposal {
   double u3;
 };
 
+// What the update of one site needs besides the spins: device index, reference index (the
+// Philox counter and the host-fed arrays use it), basis atom and moment.  With the implicit
+// geometry they come from the device index and the group table in LDS (s_grp holds the rows
+// of groups g0..), else from the per-site tables.
+struct Site {
+  int d, ir, a;
+  double mag;
+  espg::SiteGeo sg;
+};
+
+template <bool IMP>
+__device__ __forceinline__ Site site_info(const KArgs &a, int d, const int *s_grp, int g0) {
+  Site s;
+  s.d = d;
+  if (IMP) {
+    s.sg = espg::geo_site(a.geo, s_grp - g0, d);
+    s.ir = s.sg.ir;
+    s.a = s.sg.a;
+    s.mag = a.magb[s.a];
+  } else {
+    s.ir = a.dev2ref[d];
+    s.a = a.basis[d];
+    s.mag = a.mag[d];
+  }
+  return s;
+}
+
 // Proposal of one attempted update (identical on every lane of the site's group).
 template <int MODE, class SP>
-__device__ __forceinline__ Proposal propose(const KArgs &a, const SP &sp, int d, int r,
+__device__ __forceinline__ Proposal propose(const KArgs &a, const SP &sp, const Site &st, int r,
                                             unsigned long long sweep) {
   Proposal p;
-  p.so = sp.ld(d);
-  const int ir = a.dev2ref[d];
+  p.so = sp.ld(st.d);
+  const int ir = st.ir;
   if (MODE == M_HOSTFED) {
     const double *q = a.prop + 3 * ((size_t)r * a.N + ir);
     p.sn = {q[0], q[1], q[2]};
@@ -102,9 +136,8 @@ __device__ __forceinline__ Proposal propose(const KArgs &a, const SP &sp, int d,
     const uint4 rn = philox10(make_uint4((uint32_t)ir, (uint32_t)sweep, (uint32_t)(sweep >> 32),
                                          a.stream_base + (uint32_t)r),
                               a.key0, a.key1);
-    const double mag = a.mag[d];
-    p.sn = MODE == M_RANDOM ? propose_random(mag, u01(rn.x), u01(rn.y))
-                            : propose_const(mag, p.so, a.tilt, u01(rn.x), u01(rn.y));
+    p.sn = MODE == M_RANDOM ? propose_random(st.mag, u01(rn.x), u01(rn.y))
+                            : propose_const(st.mag, p.so, a.tilt, u01(rn.x), u01(rn.y));
     p.u3 = u01(rn.z);
   }
   return p;
@@ -139,31 +172,39 @@ __device__ __forceinline__ void bond_term(const KArgs &a, const double *crow, in
                cz * (ds.x * sj.y - ds.y * sj.x));
 }
 
-// Partial bond sums over the slots l = g, g+tps, ... of site d.  Inactive lanes return
-// zeros.  Slots go in chunks of CH: the CH partner indices of the next chunk are loaded while
-// the current chunk's CH partner gathers are in flight, so a wave waits for one index round
-// trip plus one gather round trip per chunk.  Full chunks need no masks; only the last
+// Partial bond sums over the slots l = g, g+tps, ... of site st.  Inactive lanes return
+// zeros.  Slots go in chunks of CH: the CH partner indices of the next chunk are loaded (or,
+// with the implicit geometry, computed from the (group, slot) rows staged in LDS at s_gs,
+// rows of groups g0..) while the current chunk's CH partner gathers are in flight, so a
+// wave waits for one gather round trip per chunk.  Full chunks need no masks; only the last
 // partial chunk is masked (its spare lanes re-read slot g).  Padded slots point at the zero
 // spin (index N), so they need no test; a self bond (SELF) reads the new spin.
-template <int CH, bool CMP, bool HB, bool HD, bool SELF, class SP>
-__device__ __forceinline__ void bond_sums(const KArgs &a, const SP &sp, const double *ctab, int d,
-                                          bool active, int g, const Proposal &p, double &eJ,
-                                          double &eB, double &eD) {
+template <int CH, bool CMP, bool HB, bool HD, bool SELF, bool IMP, class SP>
+__device__ __forceinline__ void bond_sums(const KArgs &a, const SP &sp, const double *ctab, const Site &st,
+                                          const int *s_gs, int g0, bool active, int g,
+                                          const Proposal &p, double &eJ, double &eB, double &eD) {
   eJ = 0.0; eB = 0.0; eD = 0.0;
   if (!active) return;
+  const int d = st.d;
   const V3 ds = sub(p.sn, p.so);
   const V3 si = sub(p.sn, ds);  // old spin as the reference recovers it (get_quant.F90:306,320)
   const int tps = a.tps;
   const int nk = (a.zmax - g + tps - 1) / tps;
   const int nfull = nk - nk % CH;
-  const double *crow = CMP ? ctab + (size_t)a.basis[d] * a.zmax * 5 : nullptr;
+  const double *crow = CMP ? ctab + (size_t)st.a * a.zmax * 5 : nullptr;
   const unsigned ud = (unsigned)d;
+  const int *grow = IMP ? s_gs + 2 * (st.sg.g - g0) * a.zmax : nullptr;
   unsigned jn[CH];   // partner indices of the next chunk
   auto load_idx = [&](int k0) {
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
       const int l = k0 + u < nk ? g + tps * (k0 + u) : g;
-      jn[u] = (unsigned)ld_off(a.nbr, ((unsigned)l * (unsigned)a.N + ud) * 4u);
+      if (IMP) {
+        const int2 e = *reinterpret_cast<const int2 *>(grow + 2 * l);
+        jn[u] = (unsigned)espg::geo_partner(a.geo, st.sg, e.x, e.y);
+      } else {
+        jn[u] = (unsigned)ld_off(a.nbr, ((unsigned)l * (unsigned)a.N + ud) * 4u);
+      }
     }
   };
   if (nk > 0) load_idx(0);
@@ -200,11 +241,10 @@ __device__ __forceinline__ double group_sum(double v, int tps) {
 
 // Local terms and the accept test of metropolis (get_quant.F90:166-178): reject iff
 // dE > 0 and exp(-beta dE) < u3.  Returns dE through *de.
-__device__ __forceinline__ bool decide(const KArgs &a, int d, const Proposal &p, double eJ,
+__device__ __forceinline__ bool decide(const KArgs &a, int b, const Proposal &p, double eJ,
                                        double eB, double eD, double beta, double *de_out) {
   const V3 ds = sub(p.sn, p.so);
   const V3 si = sub(p.sn, ds);
-  const int b = a.basis[d];
   double de = eJ;
   if (a.has_h) {
     const double *h = a.h4 + 4 * b;
@@ -234,16 +274,44 @@ __device__ __forceinline__ void decode_block(const KArgs &a, int &r, int &b) {
   }
 }
 
+// Coefficient layouts of the colour phases: per-bond tables, per-(basis, slot) table in LDS,
+// or the per-(basis, slot) table plus the implicit geometry (no per-site tables read).
+enum { LAY_BOND = 0, LAY_CMP = 1, LAY_IMP = 2 };
+
+// Dynamic LDS of a colour-phase block: [coefficient table][group rows][group records].
+__host__ __device__ inline size_t phase_lds_bytes(int lay, int nb, int zmax, int rows_max) {
+  size_t n = lay >= LAY_CMP ? (size_t)nb * zmax * 5 * sizeof(double) : 0;
+  if (lay == LAY_IMP) n += (size_t)rows_max * zmax * 2 * sizeof(int) + (size_t)rows_max * sizeof(int);
+  return n;
+}
+
 // Block b of colour phase (a.c_begin, a.c_count) of replica r (global-memory path);
 // a.tps lanes per site.  Shared by the single-context and the batched kernels.
-template <int MODE, bool CMP, bool HB, bool HD, bool SELF, int CH = kChunk>
+template <int MODE, int LAY, bool HB, bool HD, bool SELF, int CH = kChunk>
 __device__ __forceinline__ void phase_body(const KArgs &a, int r, int b, double *s_ctab) {
+  constexpr bool CMP = LAY >= LAY_CMP, IMP = LAY == LAY_IMP;
+  const int tps = a.tps;
+  const int per = kBlock / tps;                        // sites of this block
+  int g0 = 0;
+  int *s_gs = nullptr, *s_grp = nullptr;
   if (CMP) {
     const int n = a.nb * a.zmax * 5;
     for (int k = threadIdx.x; k < n; k += kBlock) s_ctab[k] = a.ctab[k];
-    __syncthreads();
   }
-  const int tps = a.tps;
+  if (IMP) {   // (group, slot) rows and group records of the groups this block's sites lie in
+    const int first = a.c_begin + b * per;
+    const int last = min(a.c_begin + a.c_count, first + per) - 1;
+    const int mt = (int)a.geo.mt.d;
+    g0 = first / mt;
+    const int nrow = last / mt - g0 + 1;
+    s_gs = reinterpret_cast<int *>(s_ctab + a.nb * a.zmax * 5);
+    s_grp = s_gs + 2 * a.rows_max * a.zmax;
+    const int n = 2 * nrow * a.zmax;
+    const int *src = a.gslot + (size_t)2 * g0 * a.zmax;
+    for (int k = threadIdx.x; k < n; k += kBlock) s_gs[k] = src[k];
+    for (int k = threadIdx.x; k < nrow; k += kBlock) s_grp[k] = a.grp[g0 + k];
+  }
+  if (CMP) __syncthreads();
   const int g = threadIdx.x & (tps - 1);
   const int k = (b * kBlock + threadIdx.x) / tps;
   const bool active = k < a.c_count;
@@ -253,19 +321,21 @@ __device__ __forceinline__ void phase_body(const KArgs &a, int r, int b, double 
   const unsigned long long sweep = a.clock->sweep + (unsigned long long)a.sweep_off;
   double *base = a.sp + (size_t)r * 3 * a.Ns;
   const GSpins sp{base, base + a.Ns, base + 2 * a.Ns};
+  // inactive lanes of a block sit on its first site, which lies in row g0
+  const Site st = site_info<IMP>(a, active ? d : a.c_begin + b * per, s_grp, g0);
   Proposal p{};
-  if (active) p = propose<MODE>(a, sp, d, r, sweep);
+  if (active) p = propose<MODE>(a, sp, st, r, sweep);
   double eJ, eB, eD;
-  bond_sums<CH, CMP, HB, HD, SELF>(a, sp, s_ctab, d, active, g, p, eJ, eB, eD);
+  bond_sums<CH, CMP, HB, HD, SELF, IMP>(a, sp, s_ctab, st, s_gs, g0, active, g, p, eJ, eB, eD);
   eJ = group_sum(eJ, tps);
   if (HB) eB = group_sum(eB, tps);
   if (HD) eD = group_sum(eD, tps);
   bool rej = false;
   if (active) {
     double de;
-    rej = decide(a, d, p, eJ, eB, eD, beta, &de);
+    rej = decide(a, st.a, p, eJ, eB, eD, beta, &de);
     if (g == 0) {
-      if (MODE == M_HOSTFED) a.accepted[(size_t)r * a.N + a.dev2ref[d]] = rej ? 0 : 1;
+      if (MODE == M_HOSTFED) a.accepted[(size_t)r * a.N + st.ir] = rej ? 0 : 1;
       if (!rej) {
         sp.x[d] = p.sn.x; sp.y[d] = p.sn.y; sp.z[d] = p.sn.z;
         a.w[(size_t)r * a.Ns + d] += de;
@@ -295,13 +365,13 @@ __device__ __forceinline__ void phase_body(const KArgs &a, int r, int b, double 
 }
 
 // One colour phase over every replica of one context.
-template <int MODE, bool CMP, bool HB, bool HD, bool SELF, int CH>
+template <int MODE, int LAY, bool HB, bool HD, bool SELF, int CH>
 __global__ __launch_bounds__(kBlock) void k_phase(KArgs a) {
   extern __shared__ double s_ctab[];
   int r, b;
   decode_block(a, r, b);
   if (r >= a.R) return;  // uniform per block
-  phase_body<MODE, CMP, HB, HD, SELF, CH>(a, r, b, s_ctab);
+  phase_body<MODE, LAY, HB, HD, SELF, CH>(a, r, b, s_ctab);
 }
 
 struct LdsMeas {
@@ -334,7 +404,7 @@ __device__ __forceinline__ void lds_body(const KArgs &a, int r, int nsweeps, con
   const int tps = a.tps;
   const int g = threadIdx.x & (tps - 1);
   const int per = kLdsBlock / tps;
-  unsigned int nrej = 0;
+  unsigned long long nrej = 0;   // 64-bit: one launch may run many sweeps at high rejection
   // The estimator sums of this replica's slot live in thread 0's registers during the
   // launch (same additions, same order as adding to memory each time) and are stored once
   // at the end: the sweep loop itself writes nothing but LDS.
@@ -352,16 +422,17 @@ __device__ __forceinline__ void lds_body(const KArgs &a, int r, int nsweeps, con
         const int d0 = base + threadIdx.x / tps;
         const bool active = d0 < end;
         const int d = active ? d0 : beg;
+        const Site st = site_info<false>(a, d, nullptr, 0);
         Proposal p{};
-        if (active) p = propose<MODE>(a, sp, d, r, sweep);
+        if (active) p = propose<MODE>(a, sp, st, r, sweep);
         double eJ, eB, eD;
-        bond_sums<kChunk, CMP, HB, HD, SELF>(a, sp, ctab, d, active, g, p, eJ, eB, eD);
+        bond_sums<kChunk, CMP, HB, HD, SELF, false>(a, sp, ctab, st, nullptr, 0, active, g, p, eJ, eB, eD);
         eJ = group_sum(eJ, tps);
         if (HB) eB = group_sum(eB, tps);
         if (HD) eD = group_sum(eD, tps);
         if (active) {
           double de;
-          const bool rej = decide(a, d, p, eJ, eB, eD, beta, &de);
+          const bool rej = decide(a, st.a, p, eJ, eB, eD, beta, &de);
           if (g == 0) {
             if (rej) ++nrej;
             else {
@@ -409,7 +480,7 @@ __device__ __forceinline__ void lds_body(const KArgs &a, int r, int nsweeps, con
     const double4 s = s_sp[d];
     ox[d] = s.x; ox[a.Ns + d] = s.y; ox[2 * a.Ns + d] = s.z; gw[d] = s.w;
   }
-  unsigned int v = nrej;
+  unsigned long long v = nrej;
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(&a.rej[slot], (unsigned long long)v);
 }

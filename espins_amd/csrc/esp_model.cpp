@@ -345,6 +345,76 @@ std::vector<int> greedy_colouring(int N, const std::vector<std::vector<int>> &ad
   }
   return col;
 }
+
+// Implicit geometry (esp_geo.h) of a model whose device order groups every colour by basis
+// atom and cell residue modulo the period P.  The per-(group, slot) records are read off the
+// first site of each group; the result is accepted only if the computed partner, reference
+// index, basis atom and moment equal the explicit tables for every site and slot.
+void build_implicit(HostModel &hm, const int L[3], const int P[3], const double *mag) {
+  hm.implicit = false;
+  const int N = hm.N, nb = hm.nb, zp = hm.zmax;
+  int M[3];
+  for (int k = 0; k < 3; ++k) {
+    if (P[k] < 1 || L[k] % P[k] != 0 || P[k] > 255) return;
+    M[k] = L[k] / P[k];
+    if (M[k] > 254) return;
+  }
+  const long mt = (long)M[0] * M[1] * M[2];
+  if (mt <= 0 || N % mt != 0) return;
+  const int ngroups = (int)(N / mt);
+  if (ngroups != nb * P[0] * P[1] * P[2] || ngroups > 4096) return;
+  espg::Geo &G = hm.geo;
+  G.mt = espg::make_fastdiv((uint32_t)mt);
+  G.m0 = espg::make_fastdiv((uint32_t)M[0]);
+  G.m1 = espg::make_fastdiv((uint32_t)M[1]);
+  G.M0 = M[0]; G.M1 = M[1]; G.M2 = M[2];
+  G.P0 = P[0]; G.P1 = P[1]; G.P2 = P[2];
+  G.L0 = L[0]; G.L1 = L[1];
+  G.nb = nb; G.ngroups = ngroups; G.zmax = zp; G.N = N;
+  auto cell = [&](int i, int x[3]) {
+    const int c = i / nb;
+    x[0] = c % L[0]; x[1] = (c / L[0]) % L[1]; x[2] = c / (L[0] * L[1]);
+  };
+  hm.grp.assign(ngroups, 0);
+  hm.gslot.assign((size_t)ngroups * zp * 2, 0);
+  for (int g = 0; g < ngroups; ++g) {
+    const int i0 = hm.dev2ref[(size_t)g * mt];
+    int x[3];
+    cell(i0, x);
+    hm.grp[g] = espg::grp_pack(i0 % nb, x[0] % P[0], x[1] % P[1], x[2] % P[2]);
+    const int d0 = (int)((size_t)g * mt);
+    for (int l = 0; l < zp; ++l) {
+      const int jd = hm.nbr[(size_t)l * N + d0];
+      int *e = &hm.gslot[((size_t)g * zp + l) * 2];
+      if (jd == N) { e[0] = -1; e[1] = espg::q_pack(0, 0, 0); continue; }
+      const int j = hm.dev2ref[jd];
+      int y[3], q[3];
+      cell(j, y);
+      for (int k = 0; k < 3; ++k) {
+        const int s_i = x[k] / P[k], s_j = y[k] / P[k];
+        int v = ((s_j - s_i) % M[k] + M[k]) % M[k];
+        if (v > M[k] / 2) v -= M[k];
+        q[k] = v;
+      }
+      e[0] = (int)((jd / mt) * mt);
+      e[1] = espg::q_pack(q[0], q[1], q[2]);
+    }
+  }
+  hm.magb.assign(nb, 0.0);
+  for (int a = 0; a < nb; ++a) hm.magb[a] = mag[a];
+  // verify against the explicit tables
+  for (int d = 0; d < N; ++d) {
+    const espg::SiteGeo sg = espg::geo_site(G, hm.grp.data(), d);
+    if (sg.ir != hm.dev2ref[d] || sg.a != hm.basis[d] || hm.mag[d] != hm.magb[sg.a]) return;
+    for (int l = 0; l < zp; ++l) {
+      const int *e = &hm.gslot[((size_t)sg.g * zp + l) * 2];
+      if (espg::geo_partner(G, sg, e[0], e[1]) != hm.nbr[(size_t)l * N + d]) return;
+    }
+  }
+  // groups touched by the sites of one 256-lane block (one lane per site at most)
+  hm.rows_max = (int)std::min<long>(ngroups, (256 - 1) / mt + 2);
+  hm.implicit = true;
+}
 }  // namespace
 
 std::string prepare_model(HostModel &hm, int N, int nb, const double *mag, int zJ,
@@ -444,9 +514,9 @@ std::string prepare_model(HostModel &hm, int N, int nb, const double *mag, int z
   for (int c = 0; c < ncol; ++c) hm.col_off[c + 1] += hm.col_off[c];
   hm.dev2ref.resize(N);
   hm.ref2dev.resize(N);
+  int P[3] = {1, 1, 1};
   {
     std::vector<int64_t> key(N);
-    int P[3] = {1, 1, 1};
     if (Lc[0] > 0) colour_period(N, nb, Lc, hm.colour, P);
     for (int i = 0; i < N; ++i) {
       int64_t res = 0, sub = i / nb;
@@ -523,6 +593,11 @@ std::string prepare_model(HostModel &hm, int N, int nb, const double *mag, int z
     }
   }
   if (nb > 255) return "more than 255 basis atoms";
+  if (comp && Lc[0] > 0) {
+    std::vector<double> mb(nb, 0.0);
+    for (int a = 0; a < nb; ++a) mb[a] = mag[a];   // site a of cell 0 (reference order)
+    build_implicit(hm, Lc, P, mb.data());
+  }
   return "";
 }
 

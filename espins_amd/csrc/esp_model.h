@@ -11,6 +11,8 @@
 #include <string>
 #include <vector>
 
+#include "esp_geo.h"
+
 namespace esp {
 
 // Coefficients of one merged bond: J, B, D (3).  Summed over duplicates of a pair.
@@ -32,6 +34,15 @@ struct HostModel {
   std::vector<double> mag;      // [N] device order
   std::vector<uint8_t> basis;   // [N] device order
   std::vector<double> si4, h4;  // [nb][4]: parameter, axis x, y, z
+  // Implicit geometry (esp_geo.h): partner and reference indices computed from the device
+  // index.  Set only when the lattice is periodic, the coefficients are per (basis, slot) and
+  // the computed indices equal the explicit tables for every site and slot.
+  bool implicit = false;
+  espg::Geo geo{};
+  std::vector<int> grp;         // [ngroups] packed basis atom and residue
+  std::vector<int> gslot;       // [ngroups][zmax][2]: partner group start, packed shift
+  std::vector<double> magb;     // [nb] moment of each basis atom
+  int rows_max = 0;             // groups one 256-lane block of a colour phase can touch
 };
 
 // Build the device-ready model from reference-layout tables.  Returns "" on success,

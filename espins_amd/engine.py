@@ -111,6 +111,11 @@ class Engine:
         """Lanes per site in the sweep kernels (1, 2, 4, 8; 0 = automatic)."""
         self._chk(E.lib.esp_set_tps(self._h, int(tps)))
 
+    def set_indexing(self, mode):
+        """0: partner indices computed from the lattice geometry when it allows; 1: read from the
+        per-site neighbour tables."""
+        self._chk(E.lib.esp_set_indexing(self._h, int(mode)))
+
     def coloring(self):
         n = C.c_int()
         col = np.zeros(self.N, np.int32)
@@ -121,7 +126,8 @@ class Engine:
         out = np.zeros(6, np.int32)
         self._chk(E.lib.esp_get_model_info(self._h, E.ptr(out, C.c_int)))
         return dict(N=int(out[0]), zmax=int(out[1]), ncol=int(out[2]), compressed=bool(out[3]),
-                    lds=bool(out[4]), tps=int(out[5]))
+                    layout=("bond", "basis", "implicit")[int(out[3])], lds=bool(out[4]),
+                    tps=int(out[5]))
 
     # ---- replicas --------------------------------------------------------------------
     def set_replicas(self, tems, spins):
@@ -383,7 +389,8 @@ def plan_model(m: Model):
                                  zD, E.ptr(dc, i), E.ptr(dn, i), E.ptr(dv, d), E.ptr(Ls, i),
                                  C.byref(n), E.ptr(col, i), E.ptr(info, i)))
     return n.value, col, dict(N=int(info[0]), zmax=int(info[1]), ncol=int(info[2]),
-                              compressed=bool(info[3]))
+                              compressed=bool(info[3]),
+                              layout=("bond", "basis", "implicit")[int(info[3])])
 
 
 class Batch:

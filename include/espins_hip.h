@@ -94,10 +94,17 @@ int esp_set_path(esp_ctx *ctx, int path);
 int esp_set_stream_base(esp_ctx *ctx, uint32_t base);
 /* colour[N]: 0-based colour of 1-based site i at colour[i-1]; *ncol colours. */
 int esp_get_coloring(esp_ctx *ctx, int *ncol, int *colour);
-/* Model facts: out[0]=N, out[1]=merged bond width, out[2]=colours, out[3]=1 if the
- * couplings were compressed to per-basis tables, out[4]=1 if the LDS-resident sweep is used,
- * out[5]=lanes per site in the sweep kernels. */
+/* Model facts: out[0]=N, out[1]=merged bond width, out[2]=colours, out[3]=coefficient
+ * layout of the colour phases (0 per-bond tables, 1 per-basis tables, 2 per-basis tables and
+ * the implicit lattice geometry), out[4]=1 if the LDS-resident sweep is used, out[5]=lanes per
+ * site in the sweep kernels. */
 int esp_get_model_info(esp_ctx *ctx, int *out6);
+/* Neighbour indexing of the global-memory colour phases: 0 = compute partner and reference
+ * site indices from the device index when the supercell is periodic and translation
+ * invariant (the geometry is verified against the tables for every site and slot first),
+ * 1 = always read the per-site neighbour tables.  The replaced loads are those of
+ * jij_nbors_matrix in delta_energy_get (get_quant.F90:288-336). */
+int esp_set_indexing(esp_ctx *ctx, int mode);
 /* Lanes cooperating on one site's bond sum in the sweep kernels: 1, 2, 4 or 8; 0 = automatic
  * (from the bond width). */
 int esp_set_tps(esp_ctx *ctx, int tps);

@@ -69,12 +69,15 @@ def test_energy_matches_oracle(En, name, m):
         assert abs(E[r] - Eo) <= 1e-12 * max(abs(Eo), 1.0) + 1e-14 * scale, (r, E[r], Eo)
 
 
-@pytest.mark.parametrize("path", [1, 2])
+@pytest.mark.parametrize("path,indexing", [(1, 0), (1, 1), (2, 0)],
+                         ids=["global-implicit", "global-tables", "lds"])
 @pytest.mark.parametrize("name,m", MODELS, ids=[n for n, _ in MODELS])
-def test_hostfed_decisions_bit_exact(En, name, m, path):
+def test_hostfed_decisions_bit_exact(En, name, m, path, indexing):
     """Deterministic mode: same proposals and uniforms in the same colour order give the same
-    accept/reject decisions and bit-identical spins."""
-    say("hostfed", name, path)
+    accept/reject decisions and bit-identical spins.  The global-memory phases run with partner
+    indices computed from the lattice geometry (where the lattice allows it) and with the
+    neighbour tables."""
+    say("hostfed", name, path, indexing)
     R, nsw = 3, 4
     tems = np.array([0.3, 1.0, 3.0])
     spins = Lt.random_spins(m.N, R, 11, m.mag)
@@ -86,6 +89,11 @@ def test_hostfed_decisions_bit_exact(En, name, m, path):
             eng.set_path(path)
         except Exception:
             pytest.skip("replica does not fit in LDS")
+        eng.set_indexing(indexing)
+        if path == 1 and indexing == 0 and name in ("sc_j1", "fcc_j123_si", "sc_jbd_si", "bcc_j12"):
+            assert eng.model_info()["layout"] == "implicit"
+        if indexing == 1:
+            assert eng.model_info()["layout"] != "implicit"
         eng.set_replicas(tems, spins)
         off, sites, col = colour_order(eng)
         assert o.check_colouring(col)
