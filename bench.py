@@ -133,7 +133,43 @@ def _cpu_worker_c3(args):
     while time.perf_counter() - t0 < seconds:
         e, _ = o.sweep_ref(s, 1.0 / T, 0, 0.125, st, 1, e)
         n += 1
-    return n * m.N, time.perf_counter() - t0
+    return (kind, L), n * m.N, time.perf_counter() - t0
+
+
+def cpu_baseline_c3(chains, seconds):
+    """The oracle's reference-order sweep on the batch's own update mix: every lattice class
+    (kind, L) of the batch is timed on host cores (one process per core, all cores busy at
+    once, a chain of that class at a mid-ladder temperature), and the batch's CPU throughput is
+    the update-weighted harmonic mean of the class rates times the cores -- the rate at which
+    the host cores would complete one sweep of every replica of every chain."""
+    import multiprocessing as mp
+    ncores = max(1, min(len(os.sched_getaffinity(0)), 16))
+    share, first = {}, {}
+    for i, (kind, L, J2, R, tems) in enumerate(chains):
+        nb = {"sc": 1, "bcc": 2, "fcc": 4}[kind]
+        share[(kind, L)] = share.get((kind, L), 0) + nb * L ** 3 * R
+        first.setdefault((kind, L), (kind, L, J2, float(tems[len(tems) // 2]), seconds, 100 + i))
+    classes = sorted(share, key=lambda c: -share[c])
+    jobs = [first[c] for c in classes]
+    while len(jobs) % ncores:          # keep every core busy in every round
+        jobs.append(first[classes[len(jobs) % len(classes)]])
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(ncores) as pool:
+        res = pool.map(_cpu_worker_c3, jobs)
+    wall = time.perf_counter() - t0
+    rate = {}
+    for c, u, t in res:
+        rate.setdefault(c, []).append(u / t)
+    rate = {c: sum(v) / len(v) for c, v in rate.items()}
+    total = sum(share.values())
+    per_core = total / sum(share[c] / rate[c] for c in classes)
+    return dict(value=per_core * ncores, unit="updates/s", cores=ncores, kind="port",
+                per_core=per_core,
+                sample=(f"oracle reference-order Metropolis (MT19937, sites 1..N, FP64) on every one of "
+                        f"the batch's {len(classes)} lattice classes (one process per core, {ncores} cores "
+                        f"busy at once, {seconds:.1f} s per class), combined as the update-weighted "
+                        f"harmonic mean of the class rates x cores ({wall:.1f} s wall)"),
+                class_rates={f"{k}{L}": r for (k, L), r in rate.items()})
 
 
 def run_batch(args):
@@ -143,19 +179,11 @@ def run_batch(args):
     from espins_amd import lattice as Lt
     chains = Lt.config_c3(n_chains=args.chains, seed=7)
     cpu = None
-    if not args.no_cpu:
-        ncores = max(1, min(len(os.sched_getaffinity(0)), 16))
-        jobs = [(k, L, J2, float(tems[len(tems) // 2]), args.cpu_seconds, 100 + i)
-                for i, (k, L, J2, R, tems) in enumerate(chains[:ncores])]
-        log(f"cpu_baseline: {len(jobs)} chains of the batch, {args.cpu_seconds:.0f} s per core ...")
-        with mp.get_context("fork").Pool(len(jobs)) as pool:
-            res = pool.map(_cpu_worker_c3, jobs)
-        upd = sum(r[0] for r in res)
-        cpu = dict(value=upd / max(r[1] for r in res), unit="updates/s", cores=len(jobs), kind="port",
-                   sample=(f"oracle reference-order Metropolis (MT19937, FP64) on the first {len(jobs)} "
-                           f"chains of the batch (one chain, one temperature per core), "
-                           f"{args.cpu_seconds:.0f} s per core"))
-        log(f"cpu_baseline: {cpu['value'] / 1e6:.2f} M updates/s on {len(jobs)} cores")
+    if not args.no_cpu:    # before any GPU call (forked workers must not inherit a GPU context)
+        log(f"cpu_baseline: every lattice class of the batch, {args.cpu_seconds / 4:.1f} s each ...")
+        cpu = cpu_baseline_c3(chains, args.cpu_seconds / 4)
+        log(f"cpu_baseline: {cpu['value'] / 1e6:.2f} M updates/s on {cpu['cores']} cores "
+            f"({cpu['per_core'] / 1e6:.2f} M/s per core)")
     from espins_amd.engine import Batch
     t0 = time.perf_counter()
     b = Batch(0)

@@ -108,6 +108,7 @@ _SIGS.update({
     "esp_batch_run": (_i, [_p, _i, _i, _i, _i, _i, _i, _i, _d, _u64, _u64, _u64, _i]),
     "esp_batch_num_chains": (_i, [_p]),
     "esp_batch_info": (_i, [_p, _i, _ip]),
+    "esp_batch_time_sweeps": (_i, [_p, _i, _i, _u64, _u64, _dp, _ip, _dp, _i64p]),
     "esp_batch_get_stats": (_i, [_p, _i, _dp, _i64p, _dp, _i64p]),
     "esp_batch_get_map": (_i, [_p, _i, _ip, _i64p]),
     "esp_batch_get_spins": (_i, [_p, _i, _i, _dp]),

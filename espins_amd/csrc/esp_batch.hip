@@ -192,6 +192,7 @@ struct esp_batch {
   std::vector<Var> vars;
   hipGraphExec_t gexec = nullptr;
   std::string gkey;
+  unsigned long long tune_sweep = 0;   // Philox sweeps used by the autotuner (its own range)
 };
 
 namespace {
@@ -263,11 +264,37 @@ KArgs chain_args(const Chain &c, int k) {
   a.si4 = c.si4; a.h4 = c.h4; a.beta_slot = c.beta; a.rep_slot = c.rep_slot; a.rej = c.rej;
   a.N = c.hm.N; a.Ns = c.Ns; a.nb = c.hm.nb; a.zmax = c.hm.zmax; a.R = c.R; a.ncol = c.hm.ncol;
   a.has_si = c.hm.has_si; a.has_h = c.hm.has_h; a.tps = c.tps;
+  a.mpart_nw = c.mpart_nw;
   a.stream_base = (uint32_t)k << 16;
   a.geo = c.hm.geo;
   a.grp = c.grp; a.gslot = c.gslot; a.magb = c.magb;
   a.rows_max = c.hm.rows_max;
   return a;
+}
+
+// Per-wave sums of a chain's colour phases -- spins (mpart, rewritten by every sweep) and
+// accepted dE (w, accumulated) -- laid out for its current lanes per site.  Before the
+// accumulators are re-laid out their sums move into e_base.
+int size_wave_sums(esp_batch *b, Chain &c) {
+  int nw = 1;
+  for (int col = 0; col < c.hm.ncol; ++col)
+    nw = std::max(nw, ((c.hm.col_off[col + 1] - c.hm.col_off[col]) * c.tps + 63) / 64);
+  const size_t n = (size_t)c.R * c.hm.ncol * nw;
+  if (nw == c.mpart_nw && c.mpart && c.w) {
+    BCHK(b, hipMemsetAsync(c.mpart, 0, 3 * n * sizeof(double), b->stream));
+    return ESP_OK;
+  }
+  if (c.w && c.mpart_nw > 0) {
+    hipLaunchKernelGGL(k_fold_w, dim3((c.R + 255) / 256), dim3(256), 0, b->stream, c.w,
+                       c.hm.ncol * c.mpart_nw, c.R, c.ebase);
+    BCHK(b, hipGetLastError());
+  }
+  int rc;
+  if ((rc = balloc(b, c.mpart, 3 * n)) || (rc = balloc(b, c.w, n))) return rc;
+  BCHK(b, hipMemsetAsync(c.mpart, 0, 3 * n * sizeof(double), b->stream));
+  BCHK(b, hipMemsetAsync(c.w, 0, n * sizeof(double), b->stream));
+  c.mpart_nw = nw;
+  return ESP_OK;
 }
 
 // Build device descriptor tables and the per-colour tile lists for the current plan.
@@ -284,23 +311,10 @@ int build_plan(esp_batch *b, double tilt, uint64_t seed) {
   b->max_red_blocks = 1;
   for (int k = 0; k < nc; ++k) {
     Chain &c = b->chains[k];
-    if (!c.lds) {   // per-wave spin sums for this plan's lanes per site, zeroed
-      int nw = 1;
-      for (int col = 0; col < c.hm.ncol; ++col)
-        nw = std::max(nw, ((c.hm.col_off[col + 1] - c.hm.col_off[col]) * c.tps + 63) / 64);
-      const size_t n = (size_t)c.R * c.hm.ncol * nw * 3;
-      if (nw != c.mpart_nw || !c.mpart) {
-        bfree(c.mpart);
-        if (hipMalloc((void **)&c.mpart, n * sizeof(double)) != hipSuccess) {
-          c.mpart = nullptr;
-          return bfail(b, ESP_ENOMEM, "hipMalloc failed");
-        }
-        c.mpart_nw = nw;
-      }
-      BCHK(b, hipMemsetAsync(c.mpart, 0, n * sizeof(double), b->stream));
-    }
+    int rc0;
+    if ((rc0 = size_wave_sums(b, c))) return rc0;
     ka[k] = chain_args(c, k);
-    if (!c.lds) { ka[k].mpart = c.mpart; ka[k].mpart_nw = c.mpart_nw; }
+    if (!c.lds) ka[k].mpart = c.mpart;     // spin sums only where colour phases run
     ka[k].clock = b->d_clock;
     ka[k].tilt = tilt;
     ka[k].key0 = (uint32_t)seed; ka[k].key1 = (uint32_t)(seed >> 32);
@@ -614,9 +628,7 @@ int esp_batch_commit(esp_batch *b) {
     }
     // replicas
     if ((rc = balloc(b, c.sp, (size_t)3 * R * c.Ns))) return rc;
-    if ((rc = balloc(b, c.w, (size_t)R * c.Ns))) return rc;
     BCHK(b, hipMemsetAsync(c.sp, 0, sizeof(double) * 3 * R * c.Ns, b->stream));
-    BCHK(b, hipMemsetAsync(c.w, 0, sizeof(double) * R * c.Ns, b->stream));
     std::vector<double> beta(R);
     for (int t = 0; t < R; ++t) beta[t] = 1.0 / c.tems[t];
     std::vector<int> id(R);
@@ -639,8 +651,9 @@ int esp_batch_commit(esp_batch *b) {
     if ((rc = bupload(b, d_in, c.spins.data(), c.spins.size()))) return rc;
     const size_t n = (size_t)N * R;
     hipLaunchKernelGGL(k_put_spins, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream,
-                       (const double *)d_in, (const int *)c.ref2dev, N, c.Ns, R, c.sp, c.w);
+                       (const double *)d_in, (const int *)c.ref2dev, N, c.Ns, R, c.sp);
     BCHK(b, hipGetLastError());
+    if ((rc = size_wave_sums(b, c))) return rc;
     if ((rc = chain_energy(b, k, c.ebase))) return rc;
     BCHK(b, hipMemcpyAsync(c.erun, c.ebase, sizeof(double) * R, hipMemcpyDeviceToDevice, b->stream));
     BCHK(b, hipStreamSynchronize(b->stream));
@@ -699,6 +712,8 @@ int esp_batch_run(esp_batch *b, int n_rounds, int steps_per_round, int steps_mea
   return ESP_OK;
 }
 
+}  // extern "C"
+
 // Autotuner -------------------------------------------------------------------------------
 namespace {
 std::string shape_key(const Chain &c) {
@@ -715,14 +730,24 @@ int time_chain(esp_batch *b, int k, bool lds, int tps, int nsweeps, int mode, do
   KArgs a = chain_args(c, k);
   a.tps = tps;
   a.clock = b->d_clock;
+  // the trial's per-wave accepted-dE sums go to a scratch buffer laid out for its lanes per
+  // site (the chain's own state is restored after tuning); no spin sums
+  int nw = 1;
+  for (int col = 0; col < c.hm.ncol; ++col)
+    nw = std::max(nw, ((c.hm.col_off[col + 1] - c.hm.col_off[col]) * tps + 63) / 64);
+  double *wscr = nullptr;
   a.tilt = 0.125;
   a.key0 = 0x7475u; a.key1 = 0x6e65u;
   int rc;
+  if ((rc = balloc(b, wscr, (size_t)c.R * c.hm.ncol * nw))) return rc;
+  BCHK(b, hipMemsetAsync(wscr, 0, sizeof(double) * c.R * c.hm.ncol * nw, b->stream));
+  a.w = wscr;
+  a.mpart = nullptr;
+  a.mpart_nw = nw;
   KArgs *d_one = nullptr;
   if ((rc = bupload(b, d_one, &a, 1))) return rc;
-  static unsigned long long tune_sweep = 0;
-  Clock h{(1ull << 62) + tune_sweep, 0, 1, 0};
-  tune_sweep += (unsigned long long)nsweeps;
+  Clock h{(1ull << 62) + b->tune_sweep, 0, 1, 0};
+  b->tune_sweep += (unsigned long long)nsweeps;
   BCHK(b, hipMemcpyAsync(b->d_clock, &h, sizeof(Clock), hipMemcpyHostToDevice, b->stream));
   hipEvent_t e0, e1;
   BCHK(b, hipEventCreate(&e0));
@@ -774,9 +799,9 @@ int time_chain(esp_batch *b, int k, bool lds, int tps, int nsweeps, int mode, do
   bfree(items);
   bfree(d_m);
   bfree(d_one);
+  bfree(wscr);
   return ESP_OK;
 }
-}  // namespace
 
 // Mean time of one round (one sweep of every chain + exchange) of the current plan, after one
 // untimed round.  Advances the chains; the tuning clock keeps its own Philox counters.
@@ -800,13 +825,78 @@ int time_rounds(esp_batch *b, int nrounds, double *ms) {
   *ms = t / nrounds;
   return rc;
 }
+}  // namespace
 
-int esp_batch_autotune(esp_batch *b, int nsweeps, const char *cache_path) {
-  int rc_reset;
+// The autotuner times real sweeps and exchange rounds on the chains' own buffers; the chain
+// state it changes (spins, energy accumulators, running energies, replica <-> slot labels) is
+// copied aside first and put back afterwards, so tuning never changes a trajectory.
+namespace {
+struct ChainSnap {
+  double *sp = nullptr, *ebase = nullptr, *erun = nullptr;
+  int *slot_rep = nullptr, *rep_slot = nullptr;
+};
+
+// Save (save = true) or restore the state of every chain.  Saving first folds the per-wave
+// accepted-dE accumulators into e_base (tuning may re-lay them out); restoring zeroes them in
+// whatever layout the tuned plan left.
+int snap_copy(esp_batch *b, std::vector<ChainSnap> &s, bool save) {
+  for (size_t k = 0; k < b->chains.size(); ++k) {
+    Chain &c = b->chains[k];
+    ChainSnap &q = s[k];
+    const size_t nsp = (size_t)3 * c.R * c.Ns, R = (size_t)c.R;
+    const int ne = c.hm.ncol * c.mpart_nw;
+    if (save) {
+      int rc;
+      if ((rc = balloc(b, q.sp, nsp)) || (rc = balloc(b, q.ebase, R)) || (rc = balloc(b, q.erun, R)) ||
+          (rc = balloc(b, q.slot_rep, R)) || (rc = balloc(b, q.rep_slot, R)))
+        return rc;
+      if (c.w && ne > 0)
+        hipLaunchKernelGGL(k_fold_w, dim3((c.R + 255) / 256), dim3(256), 0, b->stream, c.w, ne, c.R, c.ebase);
+    } else if (c.w && ne > 0) {
+      BCHK(b, hipMemsetAsync(c.w, 0, sizeof(double) * R * ne, b->stream));
+    }
+    auto cp = [&](void *dst, const void *src, size_t bytes) {
+      return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, b->stream);
+    };
+    const hipError_t e[5] = {
+        save ? cp(q.sp, c.sp, nsp * 8) : cp(c.sp, q.sp, nsp * 8),
+        save ? cp(q.ebase, c.ebase, R * 8) : cp(c.ebase, q.ebase, R * 8),
+        save ? cp(q.erun, c.erun, R * 8) : cp(c.erun, q.erun, R * 8),
+        save ? cp(q.slot_rep, c.slot_rep, R * 4) : cp(c.slot_rep, q.slot_rep, R * 4),
+        save ? cp(q.rep_slot, c.rep_slot, R * 4) : cp(c.rep_slot, q.rep_slot, R * 4)};
+    for (hipError_t x : e)
+      if (x != hipSuccess) return bfail(b, ESP_EHIP, std::string("autotune snapshot: ") + hipGetErrorString(x));
+  }
+  BCHK(b, hipStreamSynchronize(b->stream));
+  return ESP_OK;
+}
+
+void snap_free(std::vector<ChainSnap> &s) {
+  for (auto &q : s) {
+    bfree(q.sp); bfree(q.ebase); bfree(q.erun); bfree(q.slot_rep); bfree(q.rep_slot);
+  }
+}
+
+int autotune_body(esp_batch *b, int nsweeps, const char *cache_path);
+}  // namespace
+
+extern "C" int esp_batch_autotune(esp_batch *b, int nsweeps, const char *cache_path) {
   if (!b) return ESP_EINVAL;
   if (!b->committed) return bfail(b, ESP_ESTATE, "commit the batch before tuning");
   if (nsweeps < 1) return bfail(b, ESP_EINVAL, "nsweeps must be >= 1");
   BCHK(b, hipSetDevice(b->device));
+  std::vector<ChainSnap> snap(b->chains.size());
+  int rc = snap_copy(b, snap, true);
+  if (rc == ESP_OK) rc = autotune_body(b, nsweeps, cache_path);
+  if (rc == ESP_OK) rc = snap_copy(b, snap, false);
+  (void)hipStreamSynchronize(b->stream);
+  snap_free(snap);
+  if (rc) return rc;
+  return esp_batch_reset_stats(b);
+}
+
+namespace {
+int autotune_body(esp_batch *b, int nsweeps, const char *cache_path) {
   std::map<std::string, std::pair<int, int>> decided;  // key -> (lds, tps)
   std::map<std::string, int> lds_tps;                   // key -> best lanes per site in LDS
   if (cache_path) {
@@ -905,11 +995,66 @@ int esp_batch_autotune(esp_batch *b, int nsweeps, const char *cache_path) {
       }
     }
   }
-  if ((rc_reset = esp_batch_reset_stats(b))) return rc_reset;
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   b->gexec = nullptr;
   b->gkey.clear();
   BCHK(b, hipStreamSynchronize(b->stream));
+  return ESP_OK;
+}
+}  // namespace
+
+extern "C" {
+
+// HIP-event timing of the batched colour phases (the dominant kernel): nsweeps sweeps of the
+// global-memory chains, one event pair recorded by the dispatch packets around every launch.
+int esp_batch_time_sweeps(esp_batch *b, int nsweeps, int mode, uint64_t seed, uint64_t sweep0,
+                          double *kernel_ms, int *nlaunch, double *wall_ms, int64_t *updates) {
+  if (!b || !kernel_ms || !nlaunch) return ESP_EINVAL;
+  if (!b->committed) return bfail(b, ESP_ESTATE, "batch not committed");
+  if (nsweeps < 1 || (mode != ESP_MODE_RANDOM && mode != ESP_MODE_CONST))
+    return bfail(b, ESP_EINVAL, "bad esp_batch_time_sweeps arguments");
+  BCHK(b, hipSetDevice(b->device));
+  int rc = build_plan(b, 0.125, seed);
+  if (rc) return rc;
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+  b->gkey.clear();
+  Clock h{sweep0, 0, 1, 0};
+  BCHK(b, hipMemcpyAsync(b->d_clock, &h, sizeof(Clock), hipMemcpyHostToDevice, b->stream));
+  int per = 0;
+  for (int col = 0; col < b->maxcol; ++col)
+    for (auto &v : b->vars) per += v.ntiles[col] > 0;
+  if (per == 0) return bfail(b, ESP_ESTATE, "no global-memory chains to time");
+  std::vector<hipEvent_t> ev(2 * (size_t)per * nsweeps, nullptr);
+  for (auto &e : ev) BCHK(b, hipEventCreate(&e));
+  int k = 0;
+  for (int s = 0; s < nsweeps; ++s)
+    for (int col = 0; col < b->maxcol; ++col)
+      for (auto &v : b->vars)
+        if (v.ntiles[col] > 0) {
+          hipExtLaunchKernelGGL(bphase_fn(mode, v.variant), dim3(v.ntiles[col]), dim3(kBlock), v.shm_phase,
+                                b->stream, ev[2 * k], ev[2 * k + 1], 0, (const KArgs *)b->d_chains,
+                                (const int4 *)v.d_tiles[col], col, s);
+          ++k;
+        }
+  BCHK(b, hipGetLastError());
+  BCHK(b, hipStreamSynchronize(b->stream));
+  double sum = 0.0;
+  for (int q = 0; q < k; ++q) {
+    float ms = 0.f;
+    BCHK(b, hipEventElapsedTime(&ms, ev[2 * q], ev[2 * q + 1]));
+    sum += ms;
+  }
+  float wall = 0.f;
+  BCHK(b, hipEventElapsedTime(&wall, ev[0], ev[2 * k - 1]));
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  int64_t upd = 0;
+  for (auto &c : b->chains)
+    if (!c.lds) upd += (int64_t)c.hm.N * c.R;
+  *kernel_ms = sum;
+  *nlaunch = k;
+  if (wall_ms) *wall_ms = wall;
+  if (updates) *updates = upd * nsweeps;
   return ESP_OK;
 }
 
@@ -933,7 +1078,7 @@ int esp_batch_get_stats(esp_batch *b, int k, double *acc, int64_t *rejected, dou
     KArgs a = chain_args(c, k);
     hipLaunchKernelGGL(k_reduce, dim3(c.red_blocks, c.R), dim3(kBlock), 0, b->stream,
                        (const double *)a.sp, (const double *)a.w, a.N, a.Ns, c.red_blocks, c.partial,
-                       (const double *)nullptr, 0);
+                       (const double *)nullptr, c.hm.ncol * c.mpart_nw);
     FinArgs f{};
     f.partial = c.partial; f.nblk = c.red_blocks; f.N = c.hm.N; f.R = c.R;
     f.e_base = c.ebase; f.e_run = c.erun; f.acc = c.acc; f.nmeas = c.nmeas;

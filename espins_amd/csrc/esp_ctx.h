@@ -52,7 +52,7 @@ struct esp_ctx {
   double *d_prop = nullptr, *d_unif = nullptr;
   uint8_t *d_accd = nullptr;
   // per-wave spin sums of the colour phases: [R][ncol][mpart_nw][3]; valid while the last
-  // change of the spins was a full global-path sweep
+  // change of the spins was a full global-path sweep (d_w has the same wave layout)
   double *d_mpart = nullptr;
   int mpart_nw = 0;
   bool mpart_valid = false;
@@ -150,6 +150,7 @@ KArgs base_args(esp_ctx *c) {
   const esp::HostModel &m = c->hm;
   a.sp = c->d_sp;
   a.w = c->d_w;
+  a.mpart_nw = c->mpart_nw;
   a.tps = c->tps;
   a.nbr = c->d_nbr;
   a.cJ = c->d_cJ; a.cB = c->d_cB; a.cD = c->d_cD; a.ctab = c->d_ctab;

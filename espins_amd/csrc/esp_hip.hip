@@ -114,20 +114,29 @@ int check_ready(esp_ctx *c, bool need_replicas) {
   return need_replicas ? ensure_mpart(c) : ESP_OK;
 }
 
-// Per-wave spin sums written by the colour phases, sized for the current lanes per site
-// and zeroed on (re)allocation: entries of waves a colour does not have stay zero.  Called
-// before any capture, never inside one.
+// Per-wave sums of the colour phases -- spins (mpart) and accepted dE (d_w) -- sized for the
+// current lanes per site and zeroed on (re)allocation: entries of waves a colour does not
+// have stay zero.  Before the accumulators are re-laid out their sums move into e_base.
+// Called before any capture, never inside one.
 int ensure_mpart(esp_ctx *c) {
   const esp::HostModel &m = c->hm;
   int need = 1;
   for (int col = 0; col < m.ncol; ++col)
     need = std::max(need, ((m.col_off[col + 1] - m.col_off[col]) * c->tps + 63) / 64);
-  if (c->d_mpart && need == c->mpart_nw) return ESP_OK;
-  drop_graph(c);   // a captured round may hold the old buffer
-  const size_t n = (size_t)c->R * m.ncol * need * 3;
-  int rc = dalloc(c, c->d_mpart, n);
+  if (c->d_mpart && c->d_w && need == c->mpart_nw) return ESP_OK;
+  drop_graph(c);   // a captured round may hold the old buffers
+  if (c->d_w && c->mpart_nw > 0) {
+    hipLaunchKernelGGL(k_fold_w, dim3((c->R + 255) / 256), dim3(256), 0, c->stream, c->d_w,
+                       m.ncol * c->mpart_nw, c->R, c->d_ebase);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  const size_t n = (size_t)c->R * m.ncol * need;
+  int rc = dalloc(c, c->d_mpart, 3 * n);
   if (rc) return rc;
-  HIPCHK(c, hipMemsetAsync(c->d_mpart, 0, n * sizeof(double), c->stream));
+  if ((rc = dalloc(c, c->d_w, n))) return rc;
+  HIPCHK(c, hipMemsetAsync(c->d_mpart, 0, 3 * n * sizeof(double), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->d_w, 0, n * sizeof(double), c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->mpart_nw = need;
   c->mpart_valid = false;
@@ -196,7 +205,7 @@ int enqueue_reduce(esp_ctx *c) {
   hipLaunchKernelGGL(k_reduce, dim3(c->red_blocks, c->R), dim3(kBlock), 0, c->stream,
                      (const double *)c->d_sp, (const double *)c->d_w, c->hm.N, c->Ns,
                      c->red_blocks, c->d_partial, mp ? (const double *)c->d_mpart : nullptr,
-                     c->hm.ncol * c->mpart_nw);
+                     c->hm.ncol * c->mpart_nw);   // ncol * mpart_nw entries of both
   HIPCHK(c, hipGetLastError());
   return ESP_OK;
 }
@@ -240,11 +249,12 @@ int enqueue_energy(esp_ctx *c, double *d_out) {
   return ESP_OK;
 }
 
-// e_base = direct energy, w = 0, e_run = e_base.
+// e_base = direct energy, accepted-dE accumulators = 0, e_run = e_base.
 int rebase(esp_ctx *c) {
-  int rc = enqueue_energy(c, c->d_ebase);
+  int rc = ensure_mpart(c);
   if (rc) return rc;
-  const size_t n = (size_t)c->R * c->Ns;
+  if ((rc = enqueue_energy(c, c->d_ebase))) return rc;
+  const size_t n = (size_t)c->R * c->hm.ncol * c->mpart_nw;
   hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->d_w, n, 0.0);
   c->mpart_valid = false;
   HIPCHK(c, hipMemcpyAsync(c->d_erun, c->d_ebase, sizeof(double) * c->R, hipMemcpyDeviceToDevice,
@@ -544,10 +554,8 @@ int esp_set_replicas(esp_ctx *c, int R, const double *tems, const double *spins)
   c->tems.assign(tems, tems + R);
   c->red_blocks = std::max(1, std::min(kRedBlocks, (N + 4095) / 4096));
   if ((rc = dalloc(c, c->d_sp, (size_t)3 * R * c->Ns))) return rc;
-  if ((rc = dalloc(c, c->d_w, (size_t)R * c->Ns))) return rc;
   // the padding sites (index N: the zero spin bond padding points at) stay zero
   HIPCHK(c, hipMemsetAsync(c->d_sp, 0, sizeof(double) * 3 * R * c->Ns, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->d_w, 0, sizeof(double) * R * c->Ns, c->stream));
   if ((rc = dalloc(c, c->d_beta, R))) return rc;
   if ((rc = dalloc(c, c->d_ebase, R))) return rc;
   if ((rc = dalloc(c, c->d_erun, R))) return rc;
@@ -583,7 +591,7 @@ int esp_put_spins(esp_ctx *c, const double *spins) {
   HIPCHK(c, hipMemcpyAsync(d_in, spins, sizeof(double) * 3 * N * R, hipMemcpyHostToDevice, c->stream));
   const size_t n = (size_t)N * R;
   hipLaunchKernelGGL(k_put_spins, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                     (const double *)d_in, (const int *)c->d_ref2dev, N, c->Ns, R, c->d_sp, c->d_w);
+                     (const double *)d_in, (const int *)c->d_ref2dev, N, c->Ns, R, c->d_sp);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   dfree(d_in);
@@ -654,6 +662,7 @@ int esp_sweep_hostfed(esp_ctx *c, const double *prop, const double *unif, uint8_
   PhaseFn fn = phase_fn(M_HOSTFED, c, launch_lanes(c));
   if ((rc = set_clock(c, 0, 0, 1))) return rc;
   for (int col = 0; col < m.ncol; ++col) {
+    a.col = col;
     a.c_begin = m.col_off[col];
     a.c_count = m.col_off[col + 1] - m.col_off[col];
     if (!a.c_count) continue;

@@ -4,9 +4,10 @@
 //   x[Ns], y[Ns], z[Ns]  spin components (structure of arrays: a gather of a partner spin
 //                     is three 8-byte loads whose addresses are contiguous across a wave);
 //                     index N holds a zero spin that padded bond slots point at
-//   w[Ns]             sum of the accepted dE of this site since the last rebase, so the
-//                     running energy ("by adding differences", mcarlo.F90:748) is
-//                     e_base[r] + sum_i w_i without atomics and reproducibly
+//   wpart[ncol][nw]   sum of the accepted dE of the sites of one wave of one colour phase
+//                     since the last rebase, so the running energy ("by adding differences",
+//                     mcarlo.F90:748) is e_base[r] + sum wpart, reproducibly (one writer per
+//                     entry per phase, phases ordered by launches)
 // Bonds: nbr[zmax][N] device partner indices (self-padded); coefficients per (basis, slot)
 // in LDS when the lattice is periodic, else cJ/cB[zmax][N], cD[zmax][N][3].
 #pragma once
@@ -34,7 +35,7 @@ constexpr int kLdsRed = 4 * (512 / 64);
 
 struct KArgs {
   double *sp;                    // [R][3][Ns]
-  double *w;                     // [R][Ns]
+  double *w;                     // [R][ncol][mpart_nw] per-wave accepted-dE sums
   const int *nbr;
   const double *cJ, *cB, *cD;
   const double *ctab;            // [nb][zmax][5]
@@ -274,6 +275,37 @@ __device__ __forceinline__ void decode_block(const KArgs &a, int &r, int &b) {
   }
 }
 
+// Per-wave sums of one colour phase: the accepted dE (added to the wave's accumulator, which
+// the running energy sums) and the spins after the phase (stored; a site changes only in its
+// own colour phase, so after a sweep these add up to the replica's magnetisation and the
+// measurement reads them instead of every spin).  Fixed butterflies, one writer per entry.
+// The dE sum is added with a no-return atomic (one per wave and phase, so the order of the
+// additions to an entry is that of the launches): the wave does not wait for a read.
+__device__ __forceinline__ void wave_sums(const KArgs &a, int r, int b, bool owner, bool rej, double de,
+                                          const Proposal &p, int tps) {
+  double w = owner && !rej ? de : 0.0;
+  V3 sf = {0.0, 0.0, 0.0};
+  if (owner) sf = rej ? p.so : p.sn;
+  for (int o = 1; o < 64; o <<= 1) {
+    w += __shfl_xor(w, o, 64);
+    if (a.mpart) {
+      sf.x += __shfl_xor(sf.x, o, 64);
+      sf.y += __shfl_xor(sf.y, o, 64);
+      sf.z += __shfl_xor(sf.z, o, 64);
+    }
+  }
+  // waves past the colour's last lane own no entry (mpart_nw counts the colour's waves)
+  const int wave = (b * kBlock + threadIdx.x) >> 6;
+  if ((threadIdx.x & 63) == 0 && wave * 64 < a.c_count * tps) {
+    const size_t e = ((size_t)r * a.ncol + a.col) * a.mpart_nw + wave;
+    if (w != 0.0) atomicAdd(a.w + e, w);
+    if (a.mpart) {
+      double *o = a.mpart + e * 3;
+      o[0] = sf.x; o[1] = sf.y; o[2] = sf.z;
+    }
+  }
+}
+
 // Coefficient layouts of the colour phases: per-bond tables, per-(basis, slot) table in LDS,
 // or the per-(basis, slot) table plus the implicit geometry (no per-site tables read).
 enum { LAY_BOND = 0, LAY_CMP = 1, LAY_IMP = 2 };
@@ -331,37 +363,17 @@ __device__ __forceinline__ void phase_body(const KArgs &a, int r, int b, double 
   if (HB) eB = group_sum(eB, tps);
   if (HD) eD = group_sum(eD, tps);
   bool rej = false;
+  double de = 0.0;
   if (active) {
-    double de;
     rej = decide(a, st.a, p, eJ, eB, eD, beta, &de);
     if (g == 0) {
       if (MODE == M_HOSTFED) a.accepted[(size_t)r * a.N + st.ir] = rej ? 0 : 1;
-      if (!rej) {
-        sp.x[d] = p.sn.x; sp.y[d] = p.sn.y; sp.z[d] = p.sn.z;
-        a.w[(size_t)r * a.Ns + d] += de;
-      }
+      if (!rej) { sp.x[d] = p.sn.x; sp.y[d] = p.sn.y; sp.z[d] = p.sn.z; }
     }
   }
   const unsigned long long m = __ballot(rej && g == 0);
   if ((threadIdx.x & 63) == 0 && m) atomicAdd(&a.rej[slot], (unsigned long long)__popcll(m));
-  // A site changes only in its own colour phase, so after a sweep the per-wave spin sums of
-  // its phases add up to the replica's magnetisation: the measurement then reads these
-  // instead of every spin.  Wave-local (fixed butterfly, one writer per entry): no barrier.
-  if (a.mpart) {
-    V3 sf = {0.0, 0.0, 0.0};
-    if (active && g == 0) sf = rej ? p.so : p.sn;
-    for (int o = 1; o < 64; o <<= 1) {
-      sf.x += __shfl_xor(sf.x, o, 64);
-      sf.y += __shfl_xor(sf.y, o, 64);
-      sf.z += __shfl_xor(sf.z, o, 64);
-    }
-    // waves past the colour's last lane own no entry (mpart_nw counts the colour's waves)
-    const int wave = (b * kBlock + threadIdx.x) >> 6;
-    if ((threadIdx.x & 63) == 0 && wave * 64 < a.c_count * tps) {
-      double *o = a.mpart + (((size_t)r * a.ncol + a.col) * a.mpart_nw + wave) * 3;
-      o[0] = sf.x; o[1] = sf.y; o[2] = sf.z;
-    }
-  }
+  wave_sums(a, r, b, active && g == 0, rej, de, p, tps);
 }
 
 // One colour phase over every replica of one context.
@@ -377,7 +389,7 @@ __global__ __launch_bounds__(kBlock) void k_phase(KArgs a) {
 struct LdsMeas {
   double *acc;                   // [R][6] by slot
   double *e_run;                 // [R] by replica
-  const double *e_base;
+  double *e_base;                // [R]: the launch's accepted dE is added at its end
   unsigned long long *nmeas;
   int measure_every;             // 0 = never
 };
@@ -390,13 +402,20 @@ __device__ __forceinline__ void lds_body(const KArgs &a, int r, int nsweeps, con
   double *ctab = reinterpret_cast<double *>(s_sp + a.N + 1);
   double *red = ctab + (CMP ? a.nb * a.zmax * 5 : 0);
   const double *gx = a.sp + (size_t)r * 3 * a.Ns;
-  double *gw = a.w + (size_t)r * a.Ns;
+  // w of a site in LDS: its accepted dE during this launch
   for (int d = threadIdx.x; d <= a.N; d += kLdsBlock)   // index N: the zero spin
-    s_sp[d] = d < a.N ? make_double4(gx[d], gx[a.Ns + d], gx[2 * a.Ns + d], gw[d])
+    s_sp[d] = d < a.N ? make_double4(gx[d], gx[a.Ns + d], gx[2 * a.Ns + d], 0.0)
                       : make_double4(0.0, 0.0, 0.0, 0.0);
   if (CMP)
     for (int k = threadIdx.x; k < a.nb * a.zmax * 5; k += kLdsBlock) ctab[k] = a.ctab[k];
-  __syncthreads();
+  // accepted dE the global-memory phases accumulated since the last rebase
+  double wg[1] = {0.0};
+  {
+    const int ne = a.ncol * a.mpart_nw;
+    const double *wr = a.w + (size_t)r * ne;
+    for (int k = threadIdx.x; k < ne; k += kLdsBlock) wg[0] += wr[k];
+  }
+  block_sum<1>(wg, red);     // (its barriers also order the LDS fills above)
   const LSpins sp{s_sp};
   const int slot = a.rep_slot[r];
   const double beta = a.beta_slot[slot];
@@ -414,6 +433,7 @@ __device__ __forceinline__ void lds_body(const KArgs &a, int r, int nsweeps, con
   if (threadIdx.x == 0 && m.measure_every > 0)
     for (int k = 0; k < 6; ++k) A[k] = m.acc[6 * slot + k];
   const double e_base = m.e_base[r];
+  double w_launch = 0.0;      // thread 0: the launch's accepted dE at the last block sum
   for (int sw = 0; sw < nsweeps; ++sw) {
     const unsigned long long sweep = sweep0 + (unsigned long long)sw;
     for (int c = 0; c < a.ncol; ++c) {
@@ -454,7 +474,8 @@ __device__ __forceinline__ void lds_body(const KArgs &a, int r, int nsweeps, con
       }
       block_sum<4>(v, red);
       if (threadIdx.x == 0) {
-        const double e = e_base + v[0];
+        const double e = e_base + wg[0] + v[0];
+        w_launch = v[0];
         if (meas) {
           const double n = (double)a.N;
           const double mx = v[1] / n, my = v[2] / n, mz = v[3] / n;
@@ -473,12 +494,15 @@ __device__ __forceinline__ void lds_body(const KArgs &a, int r, int nsweeps, con
       for (int k = 0; k < 6; ++k) m.acc[6 * slot + k] = A[k];
       if (r == 0) m.nmeas[0] += nmeas_local;
     }
-    if (nsweeps > 0) m.e_run[r] = e_last;
+    if (nsweeps > 0) {
+      m.e_run[r] = e_last;
+      m.e_base[r] = e_base + w_launch;   // one block per replica: the only writer
+    }
   }
   double *ox = a.sp + (size_t)r * 3 * a.Ns;
   for (int d = threadIdx.x; d < a.N; d += kLdsBlock) {
     const double4 s = s_sp[d];
-    ox[d] = s.x; ox[a.Ns + d] = s.y; ox[2 * a.Ns + d] = s.z; gw[d] = s.w;
+    ox[d] = s.x; ox[a.Ns + d] = s.y; ox[2 * a.Ns + d] = s.z;
   }
   unsigned long long v = nrej;
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -491,27 +515,27 @@ __global__ __launch_bounds__(kLdsBlock) void k_lds(KArgs a, int nsweeps, LdsMeas
   lds_body<MODE, CMP, HB, HD, SELF>(a, blockIdx.x, nsweeps, m, s_sp);
 }
 
-// Partial sums of (w, x, y, z) per (replica, block) for the running energy and m.  With
-// mpart (the colour phases' per-wave spin sums, nm = ncol * mpart_nw entries per replica)
-// the spins themselves are not read.
-__device__ __forceinline__ void reduce_body(const double *sp, const double *w, int N, int Ns, int nblk,
-                                            double *partial, const double *mpart, int nm, int r, int b) {
+// Partial sums per (replica, block) for the running energy and m: the per-wave accepted-dE
+// accumulators (ne entries per replica) and either the colour phases' per-wave spin sums
+// (mpart, ne entries of 3) or, without them, the spins.
+__device__ __forceinline__ void reduce_body(const double *sp, const double *wpart, int N, int Ns, int nblk,
+                                            double *partial, const double *mpart, int ne, int r, int b) {
   __shared__ double red[4 * (kBlock / 64)];
-  const int chunk = (N + nblk - 1) / nblk;
-  const int beg = b * chunk, end = min(N, beg + chunk);
   double v[4] = {0, 0, 0, 0};
-  const double *x = sp + (size_t)r * 3 * Ns;
-  const double *wr = w + (size_t)r * Ns;
+  const int mc = (ne + nblk - 1) / nblk;
+  const double *wr = wpart + (size_t)r * ne;
+  for (int q = b * mc + threadIdx.x; q < min(ne, (b + 1) * mc); q += kBlock) v[0] += wr[q];
   if (mpart) {
-    for (int d = beg + threadIdx.x; d < end; d += kBlock) v[0] += wr[d];
-    const int mc = (nm + nblk - 1) / nblk;
-    const double *mr = mpart + (size_t)r * nm * 3;
-    for (int q = b * mc + threadIdx.x; q < min(nm, (b + 1) * mc); q += kBlock) {
+    const double *mr = mpart + (size_t)r * ne * 3;
+    for (int q = b * mc + threadIdx.x; q < min(ne, (b + 1) * mc); q += kBlock) {
       v[1] += mr[3 * q]; v[2] += mr[3 * q + 1]; v[3] += mr[3 * q + 2];
     }
   } else {
+    const int chunk = (N + nblk - 1) / nblk;
+    const int beg = b * chunk, end = min(N, beg + chunk);
+    const double *x = sp + (size_t)r * 3 * Ns;
     for (int d = beg + threadIdx.x; d < end; d += kBlock) {
-      v[0] += wr[d]; v[1] += x[d]; v[2] += x[Ns + d]; v[3] += x[2 * Ns + d];
+      v[1] += x[d]; v[2] += x[Ns + d]; v[3] += x[2 * Ns + d];
     }
   }
   block_sum<4>(v, red);
@@ -519,9 +543,20 @@ __device__ __forceinline__ void reduce_body(const double *sp, const double *w, i
     for (int k = 0; k < 4; ++k) partial[((size_t)r * nblk + b) * 4 + k] = v[k];
 }
 
-__global__ __launch_bounds__(kBlock) void k_reduce(const double *sp, const double *w, int N, int Ns,
-                                                   int nblk, double *partial, const double *mpart, int nm) {
-  reduce_body(sp, w, N, Ns, nblk, partial, mpart, nm, blockIdx.y, blockIdx.x);
+__global__ __launch_bounds__(kBlock) void k_reduce(const double *sp, const double *wpart, int N, int Ns,
+                                                   int nblk, double *partial, const double *mpart, int ne) {
+  reduce_body(sp, wpart, N, Ns, nblk, partial, mpart, ne, blockIdx.y, blockIdx.x);
+}
+
+// e_base[r] += the sum of replica r's per-wave accumulators (fixed order); they are then zero.
+// Used before the accumulators are re-laid out.
+__global__ void k_fold_w(double *wpart, int ne, int R, double *e_base) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  double *w = wpart + (size_t)r * ne;
+  double s = 0.0;
+  for (int q = 0; q < ne; ++q) { s += w[q]; w[q] = 0.0; }
+  e_base[r] += s;
 }
 
 struct FinArgs {
@@ -674,9 +709,8 @@ __global__ void k_energy_fin(const double *partial, int nblk, int R, double *E) 
   E[r] = s / 2.0;
 }
 
-// Spins in: reference order [R][N][3] -> device order SoA, w = 0.
-__global__ void k_put_spins(const double *in, const int *ref2dev, int N, int Ns, int R, double *sp,
-                            double *w) {
+// Spins in: reference order [R][N][3] -> device order SoA.
+__global__ void k_put_spins(const double *in, const int *ref2dev, int N, int Ns, int R, double *sp) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)N * R) return;
   const int r = (int)(t / N), i = (int)(t % N);
@@ -684,7 +718,6 @@ __global__ void k_put_spins(const double *in, const int *ref2dev, int N, int Ns,
   const int d = ref2dev[i];
   double *x = sp + (size_t)r * 3 * Ns;
   x[d] = p[0]; x[Ns + d] = p[1]; x[2 * Ns + d] = p[2];
-  w[(size_t)r * Ns + d] = 0.0;
 }
 
 __global__ void k_get_spins(const double *sp, const int *ref2dev, const int *src_rep, int N, int Ns,

@@ -482,10 +482,12 @@ __global__ __launch_bounds__(kBlock) void k_phase_learned(KArgs a, const uint16_
   const bool acc = ratio >= 1.0 || u01(rn.w) < ratio;
   if (active && acc) {
     sp.x[d] = sn.x; sp.y[d] = sn.y; sp.z[d] = sn.z;
-    a.w[(size_t)r * a.Ns + d] += dE;
   }
   const unsigned long long m = __ballot(active && !acc);
   if ((threadIdx.x & 63) == 0 && m) atomicAdd(&a.rej[slot], (unsigned long long)__popcll(m));
+  Proposal pr;
+  pr.so = so; pr.sn = sn; pr.u3 = 0.0;
+  wave_sums(a, r, b, active, !acc, dE, pr, 1);   // accepted dE into the wave's accumulator
 }
 
 // Records of local conditionals: n random (replica, site) pairs of the current state.
@@ -811,12 +813,14 @@ int learn_enqueue_sweeps(esp_ctx *c, uint64_t seed, int nsweeps, int off0, hipEv
   const esp::HostModel &m = c->hm;
   KArgs a = base_args(c);
   a.tps = 1;
+  a.mpart = nullptr;                  // the learned phases keep no spin sums
   a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32);
   LearnFn fn = learn_fn(m);
   const size_t shm = (m.compressed ? c->ctab_bytes : 0) + kNP * sizeof(float);
   int k = 0;
   for (int s = 0; s < nsweeps; ++s)
     for (int col = 0; col < m.ncol; ++col) {
+      a.col = col;
       a.c_begin = m.col_off[col];
       a.c_count = m.col_off[col + 1] - m.col_off[col];
       if (a.c_count == 0) continue;

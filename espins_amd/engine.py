@@ -475,6 +475,15 @@ class Batch:
                                       int(start_parity), _mode(mode), float(tilt),
                                       int(seed) & (2**64 - 1), int(sweep0), int(step0), int(graph)))
 
+    def time_sweeps(self, nsweeps, mode="random", seed=1, sweep0=0):
+        """HIP-event timing of the batched colour phases (see esp_batch_time_sweeps)."""
+        kms, wall = C.c_double(), C.c_double()
+        nl, upd = C.c_int(), C.c_int64()
+        self._chk(E.lib.esp_batch_time_sweeps(self._h, int(nsweeps), _mode(mode), int(seed) & (2**64 - 1),
+                                              int(sweep0), C.byref(kms), C.byref(nl), C.byref(wall),
+                                              C.byref(upd)))
+        return dict(kernel_ms=kms.value, nlaunch=nl.value, wall_ms=wall.value, updates=upd.value)
+
     def info(self, k):
         out = np.zeros(8, np.int32)
         self._chk(E.lib.esp_batch_info(self._h, int(k), E.ptr(out, C.c_int)))

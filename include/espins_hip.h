@@ -247,6 +247,12 @@ int esp_batch_run(esp_batch *b, int n_rounds, int steps_per_round, int steps_mea
 int esp_batch_num_chains(esp_batch *b);
 /* out[8]: N, R, colours, lanes per site, LDS path, bond width, kernel variant, LDS fits. */
 int esp_batch_info(esp_batch *b, int chain, int *out8);
+/* HIP-event timing of the batched colour-phase kernel over nsweeps sweeps of every chain on
+ * the global-memory path (diagnostics for the roofline; no reference counterpart): summed
+ * kernel time, number of launches, wall time first start -> last end, and the attempted
+ * updates of those launches.  The chains advance (a Philox range from sweep0). */
+int esp_batch_time_sweeps(esp_batch *b, int nsweeps, int mode, uint64_t seed, uint64_t sweep0,
+                          double *kernel_ms, int *nlaunch, double *wall_ms, int64_t *updates);
 int esp_batch_get_stats(esp_batch *b, int chain, double *acc, int64_t *rejected, double *e_run,
                         int64_t *nmeas);
 int esp_batch_get_map(esp_batch *b, int chain, int *slot_rep, int64_t *nswap);

@@ -95,3 +95,41 @@ def test_autotune_picks_valid_shapes(tmp_path):
             assert b.stats(k)["nmeas"] == 3
     lines = open(cache).read().split("\n")
     assert len([l for l in lines if l.strip()]) == len(cs)
+
+
+def test_autotune_leaves_chain_state_unchanged(tmp_path):
+    """Tuning times real sweeps and exchanges on the chains' own buffers; afterwards spins,
+    labels, swap counters and running energies are bit-identical to before, with and without
+    a cache file, and a following run equals the same run of an untuned batch."""
+    from espins_amd.engine import Batch
+    cs = chains()
+    cache = str(tmp_path / "tune.txt")
+    init = [Lt.random_spins(m.N, len(t), 50 + k, m.mag) for k, (m, t) in enumerate(cs)]
+
+    def state(b):
+        return [(b.spins(k), b.pt_map(k), b.stats(k)["e_run"]) for k in range(len(cs))]
+
+    runs = []
+    for tune in (None, "fresh", "cached"):
+        with Batch() as b:
+            for (m, t), s in zip(cs, init):
+                b.add_chain(m, t, s)
+            b.commit()
+            b.run(2, measure=True, exchange=True, seed=5)
+            before = state(b)
+            if tune:
+                b.autotune(nsweeps=2, cache=cache)
+                after = state(b)
+                for x, y in zip(before, after):
+                    assert np.array_equal(x[0], y[0])
+                    assert np.array_equal(x[1][0], y[1][0])
+                    assert np.array_equal(x[2], y[2])
+            # the same kernel shapes for every run: the comparison is about the state only
+            for k in range(len(cs)):
+                b.set_chain_config(k, 1, 1)
+            b.run(3, measure=True, exchange=True, seed=5, sweep0=4, step0=2)
+            runs.append(state(b))
+    for other in runs[1:]:
+        for x, y in zip(runs[0], other):
+            assert np.array_equal(x[0], y[0])
+            assert np.array_equal(x[1][0], y[1][0])
