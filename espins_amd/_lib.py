@@ -65,6 +65,7 @@ _SIGS = {
     "esp_get_spins": (_i, [_p, _i, _dp]),
     "esp_put_spins": (_i, [_p, _dp]),
     "esp_energy": (_i, [_p, _dp]),
+    "esp_energy_noalloc": (_i, [_p, C.c_void_p]),
     "esp_sweep": (_i, [_p, _i, _i, _d, _u64, _u64]),
     "esp_sweep_hostfed": (_i, [_p, _dp, _dp, _u8p]),
     "esp_measure": (_i, [_p]),

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -23,6 +24,33 @@ using namespace espk;
 
 namespace {
 constexpr int kRedBlocks = 64;    // reduce blocks per replica (upper bound)
+
+// Scratch of the allocation-free entry point (esp_energy_noalloc): the block partials of the
+// direct energy.  A file-scope device array: every GPU the library runs on holds its own
+// copy until the process ends.  Host code never uses the array's name as an address; it
+// uses the per-device address looked up once by hipGetSymbolAddress (scratch_address).
+constexpr size_t kNoallocScratch = size_t(1) << 18;   // doubles: 2 MiB per GPU
+static __device__ __align__(256) double scratch[kNoallocScratch];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (which must be current): looked up on the first
+// call for that device, then cached.  Returns null if the lookup fails.
+double *scratch_address(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    g_scratch_dev[device] = static_cast<double *>(p);
+  }
+  return g_scratch_dev[device];
+}
 }  // namespace
 
 #include "esp_ctx.h"
@@ -337,6 +365,9 @@ int esp_create(esp_ctx **out, int device) {
     delete c;
     return ESP_ENOMEM;
   }
+  // set-up of the allocation-free entry point on this device (its scratch address); a
+  // failure here only disables esp_energy_noalloc, which then reports it
+  (void)scratch_address(device);
   *out = c;
   return ESP_OK;
 }
@@ -626,6 +657,32 @@ int esp_energy(esp_ctx *c, double *E) {
   HIPCHK(c, hipMemcpyAsync(E, d_e, sizeof(double) * c->R, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   dfree(d_e);
+  return ESP_OK;
+}
+
+int esp_energy_noalloc(esp_ctx *c, double *E_dev) {
+  if (!c) return ESP_EINVAL;
+  if (!c->have_model) return fail(c, ESP_ESTATE, "no model: call esp_set_model first");
+  if (c->R <= 0) return fail(c, ESP_ESTATE, "no replicas: call esp_set_replicas first");
+  if (!E_dev) return fail(c, ESP_EINVAL, "E is NULL");
+  const size_t need = (size_t)c->R * c->red_blocks;
+  if (need > kNoallocScratch)
+    return fail(c, ESP_EINVAL, "esp_energy_noalloc: replicas x energy blocks (" + std::to_string(need) +
+                                   ") exceed the scratch of " + std::to_string(kNoallocScratch) + " doubles");
+  HIPCHK(c, hipSetDevice(c->device));
+  double *part = scratch_address(c->device);
+  if (!part) return fail(c, ESP_EHIP, "esp_energy_noalloc: hipGetSymbolAddress of the scratch failed");
+  const esp::HostModel &m = c->hm;
+  KArgs a = base_args(c);
+  EnergyFn fn = energy_fn(m.compressed, m.has_B, m.has_D);
+  const int nblk = c->red_blocks;
+  // every partial the finishing kernel reads is written by the first: nothing to clear
+  hipLaunchKernelGGL(fn, dim3(nblk, c->R), dim3(kBlock),
+                     (m.compressed ? c->ctab_bytes : 0) + (kBlock / 64) * sizeof(double), c->stream, a,
+                     nblk, part);
+  hipLaunchKernelGGL(k_energy_fin, dim3((c->R + 255) / 256), dim3(256), 0, c->stream, (const double *)part,
+                     nblk, c->R, E_dev);
+  HIPCHK(c, hipGetLastError());
   return ESP_OK;
 }
 

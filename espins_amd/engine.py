@@ -156,6 +156,11 @@ class Engine:
         self._chk(E.lib.esp_energy(self._h, E.ptr(out, C.c_double)))
         return out
 
+    def energy_noalloc(self, dev_ptr):
+        """Enqueue the direct energies into device memory at dev_ptr (R doubles) without any
+        device allocation (esp_energy_noalloc); call sync() before reading them."""
+        self._chk(E.lib.esp_energy_noalloc(self._h, C.c_void_p(int(dev_ptr))))
+
     def sweep(self, nsweeps=1, mode="random", tilt=0.125, seed=1, sweep0=0):
         self._chk(E.lib.esp_sweep(self._h, int(nsweeps), _mode(mode), float(tilt),
                                   int(seed) & (2**64 - 1), int(sweep0)))

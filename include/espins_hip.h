@@ -122,6 +122,16 @@ int esp_put_spins(esp_ctx *ctx, const double *spins);
 /* ---- hot path -------------------------------------------------------------------- */
 /* energy_get (get_quant.F90:186-261) of every replica: E[R] (by replica). */
 int esp_energy(esp_ctx *ctx, double *E);
+/* Allocation-free variant of esp_energy, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  E is DEVICE-accessible memory of R doubles; the direct
+ * energies of the R replicas are enqueued on the context's stream (esp_stream) and the call
+ * returns without waiting.  The block partials live in a fixed device scratch array of
+ * 262144 doubles (2 MiB) per GPU, looked up at esp_create: R x energy blocks (one block per
+ * 4096 sites, at most 64) must not exceed it, else ESP_EINVAL.  The call makes no
+ * hipMalloc/hipFree.  All contexts on one GPU share that array: calls that overlap on one
+ * GPU (different streams, or threads) race. */
+int esp_energy_noalloc(esp_ctx *ctx, double *E);
 /* nsweeps Metropolis sweeps of every replica at its slot's temperature, in colour order:
  * metropolis + delta_energy_get (get_quant.F90:52-182, 264-337) with Philox4x32-10
  * uniforms, counter (site, sweep, replica), key seed.  Sweep k uses counter sweep0+k. */
