@@ -11,7 +11,11 @@ program mc_gpu
   use iso_c_binding
   use espins_hip
   use mc_input
+  use mc_rng
   implicit none
+
+  real(dp), parameter :: pi = 3.141592653589793238462643383279_dp   ! constants.F90:30
+  real(dp), parameter :: twopi = 2 * pi
 
   type(input_t) :: inp
   type(c_ptr) :: ctx
@@ -28,6 +32,9 @@ program mc_gpu
   integer :: w1, nsw_warm, nsw, steps_mc
   integer(8) :: t0, t1, rate
   real(dp) :: beta, C, chi, UE, UM, dE, dM, nrm, r1, r2
+  type(mt_t) :: rng
+  integer :: rank_seed
+  real(c_double), allocatable :: e_init(:), e_slot(:), d_slot(:)
 
   narg = command_argument_count()
   if (narg < 1) then
@@ -90,21 +97,28 @@ program mc_gpu
   allocate(tems(R), spins(3, N, R), acc(6, R), e0(R), edir(R), erun(R), rej_w(R), rej(R), &
            nswap(2, R), smap(R))
   tems = inp%tems
-  seed = merge(int(inp%seed, c_int64_t), 20240601_c_int64_t, inp%have_seed)
+  ! the rank's random seed: the seeds keyword, else /dev/urandom (comms_init_random,
+  ! comms.F90:979-1007); it seeds the host MT19937 stream and the device Philox key
+  rank_seed = inp%seed
+  if (.not. inp%have_seed) rank_seed = urandom_seed()
+  seed = int(rank_seed, c_int64_t)
+  write(out_u, '(1x,a,i12)') 'Random seed                : ', rank_seed
+  call mt_seed(rng, rank_seed)
   if (inp%initial(1:5) == 'ferro') then
     spins = 0
     do k = 1, R
       spins(3, :, k) = mag
     end do
   else if (inp%initial(1:4) == 'rand') then
-    ! the reference's 'rand' formula (not uniform on the sphere), identical for all replicas
-    call random_seed()
+    ! the reference's 'rand' configuration (mcarlo.F90:275-286, pt.F90:448-459): two draws of
+    ! the rank's MT19937 stream per site in site order, not uniform on the sphere, the same for
+    ! every replica
     do i = 1, N
-      call random_number(r1)
-      call random_number(r2)
-      spins(1, i, 1) = mag(i) * sin(r1 * acos(-1.0_dp)) * cos(r2 * 2 * acos(-1.0_dp))
-      spins(2, i, 1) = mag(i) * sin(r1 * acos(-1.0_dp)) * sin(r2 * 2 * acos(-1.0_dp))
-      spins(3, i, 1) = mag(i) * cos(r1 * acos(-1.0_dp))
+      r1 = mt_real2(rng)
+      r2 = mt_real2(rng)
+      spins(1, i, 1) = mag(i) * sin(r1 * pi) * cos(r2 * twopi)
+      spins(2, i, 1) = mag(i) * sin(r1 * pi) * sin(r2 * twopi)
+      spins(3, i, 1) = mag(i) * cos(r1 * pi)
     end do
     do k = 2, R
       spins(:, :, k) = spins(:, :, 1)
@@ -142,63 +156,114 @@ program mc_gpu
                          int(steps_mc, 8) * inp%steps_measure * N, .false.)
   else
     ! ---- parallel tempering schedule (pt.F90:529-544) ------------------------------------
+    allocate(e_init(R), e_slot(R), d_slot(R))
     w1 = ceiling(real(inp%steps_warmup, dp) / 2.0_dp)
     nsw_warm = ceiling(real(w1, dp) / real(inp%pt_steps_swap, dp))
+    if (inp%pt_steps_swap * nsw_warm /= w1) then
+      write(out_u, '(1x,a)') '***** The number of Warm-up steps is not diviable to pt_steps_swap : '
+      write(out_u, '(1x,a,I9)') '***** For consistancy the number of Warm-up steps increases to     : ', &
+                                2 * nsw_warm * inp%pt_steps_swap
+    end if
     w1 = nsw_warm * inp%pt_steps_swap
     nsw = ceiling(real(inp%steps_mc, dp) / real(inp%pt_steps_swap, dp))
+    if (nsw * inp%pt_steps_swap /= inp%steps_mc) then
+      write(out_u, '(1x,a)') '***** The number of Monte Carlo steps is not diviable to pt_steps_swap : '
+      write(out_u, '(1x,a,I9)') '***** For consistancy the number of Monte Carlo steps increases to     : ', &
+                                nsw * inp%pt_steps_swap
+    end if
     steps_mc = nsw * inp%pt_steps_swap
     sweep = 0
-    if (w1 > 0) then            ! warmup1: plain sweeps
+    if (w1 > 0) then            ! warmup1: plain sweeps (pt.F90:557-629)
       call check(esp_run(ctx, 1_c_int, int(w1, c_int), int(inp%steps_measure, c_int), 0_c_int, 0_c_int, &
                  1_c_int, mode, real(inp%tilt, c_double), seed, sweep, 0_c_int64_t, 0_c_int), 'warmup1')
       sweep = sweep + int(w1, c_int64_t) * inp%steps_measure
-      ! warmup2: swaps every pt_steps_swap MC steps, parity 1,2,1,...
-      call check(esp_run(ctx, int(nsw_warm, c_int), int(inp%pt_steps_swap, c_int), &
-                 int(inp%steps_measure, c_int), 0_c_int, 1_c_int, 1_c_int, mode, real(inp%tilt, c_double), &
-                 seed, sweep, 0_c_int64_t, 1_c_int), 'warmup2')
-      sweep = sweep + int(w1, c_int64_t) * inp%steps_measure
+      call check(esp_get_stats(ctx, acc, rej_w, erun, nmeas), 'esp_get_stats')
+      call check(esp_energy(ctx, edir), 'esp_energy')
+      write(out_u, '(/1x,a)') '*------------------------------------ WARMUP1 --------------------------------------*'
+      write(out_u, '(1x,a)') '|     Temp      Total Steps    Rejected Steps    Accepted Steps   Acc/Tot Ratio     |'
+      write(out_u, '(1x,a)') '+-----------------------------------------------------------------------------------+'
+      do k = 1, R               ! no exchanges yet: replica k is at temperature k
+        write(out_u, 224) tems(k), int(w1, 8) * inp%steps_measure * N, rej_w(k), &
+                          int(w1, 8) * inp%steps_measure * N - rej_w(k), &
+                          real(int(w1, 8) * inp%steps_measure * N - rej_w(k), dp) / &
+                          real(int(w1, 8) * inp%steps_measure * N, dp), '  <-- WARMUP1'
+      end do
+      call energy_table('WARMUP1', e0, erun, edir, 'ENERGY1')
+      call check(esp_get_spins(ctx, 1_c_int, spins), 'esp_get_spins')
+      call write_sconfig(trim(seedname)//'_sconfig_warmup.dat', .false.)
+      ! warmup2: exchanges every pt_steps_swap MC steps, parity 1,2,1,... (pt.F90:654-713)
+      call check(esp_reset_stats(ctx), 'esp_reset_stats')
+      e_init = erun
+      write(out_u, '(/1x,a)') '*---------------------------------- SWAP in WARMUP2 --------------------------------*'
+      write(out_u, '(1x,a)') '|     Num_Steps                     Configuration Numbers                           |'
+      write(out_u, '(1x,a)') '+-----------------------------------------------------------------------------------+'
       call check(esp_get_map(ctx, smap, nswap), 'esp_get_map')
-      write(out_u, '(/1x,a)') '*------------------------------- SWAP in WARMUP2 -----------------------------------*'
-      write(out_u, '(1x,a,1000i4)') 'Final configuration = ', smap
+      write(out_u, *) 'Initial configuration = ', smap
+      call pt_rounds(nsw_warm, w1, 0_c_int, 0, 'warmup2')
+      write(out_u, *) 'Final configuration = ', smap
+      write(out_u, '(1x,85a)') '*-----------------------------------------------------------------------------------*'
+      call check(esp_get_stats(ctx, acc, rej_w, erun, nmeas), 'esp_get_stats')
+      call check(esp_get_map(ctx, smap, nswap), 'esp_get_map')
+      call check(esp_energy(ctx, edir), 'esp_energy')
       call swap_table('WARMUP2')
+      write(out_u, '(/1x,a)') '*------------------------------------ WARMUP2 --------------------------------------*'
+      write(out_u, '(1x,a)') '|     Temp      Total Steps    Rejected Steps    Accepted Steps   Acc/Tot Ratio     |'
+      write(out_u, '(1x,a)') '+-----------------------------------------------------------------------------------+'
+      do k = 1, R               ! rejections are counted by temperature slot
+        write(out_u, 224) tems(k), int(w1, 8) * inp%steps_measure * N, rej_w(k), &
+                          int(w1, 8) * inp%steps_measure * N - rej_w(k), &
+                          real(int(w1, 8) * inp%steps_measure * N - rej_w(k), dp) / &
+                          real(int(w1, 8) * inp%steps_measure * N, dp), '  <-- WARMUP2'
+      end do
+      ! energies follow the configurations (by slot now); the initial column stays by slot
+      do k = 1, R
+        e_slot(k) = erun(smap(k))
+        d_slot(k) = edir(smap(k))
+      end do
+      call energy_table('WARMUP2', e_init, e_slot, d_slot, 'ENERGY2')
+      sweep = sweep + int(w1, c_int64_t) * inp%steps_measure
+    else
+      write(out_u, '(/1x,85a)') '+------------------------------------ WARMUP --------------------------------------+'
+      write(out_u, '(1x,85a)') '|                                                                                   |'
+      write(out_u, '(1x,85a)') '+-----------------------------------------------------------------------------------+'
+      write(out_u, '(1x,85a)') ' No Warm-up step is done! '
+      write(out_u, '(1x,a)') '*--------------------------------- END of WARMUP -----------------------------------*'
     end if
     if (learned) call train_proposal()
     call check(esp_get_stats(ctx, acc, rej_w, erun, nmeas), 'esp_get_stats')
+    call check(esp_get_map(ctx, smap, nswap), 'esp_get_map')
+    do k = 1, R
+      e_init(k) = erun(smap(k))   ! running energy of the configuration at each slot
+    end do
     call check(esp_reset_stats(ctx), 'esp_reset_stats')
     if (inp%energy_write) call check(esp_set_trace(ctx, int(steps_mc, c_int)), 'esp_set_trace')
     ! main phase: measurement after every MC step, exchange every pt_steps_swap steps,
-    ! parity restarts at 1 (pt.F90:898)
-    call check(esp_run(ctx, int(nsw, c_int), int(inp%pt_steps_swap, c_int), int(inp%steps_measure, c_int), &
-               1_c_int, 1_c_int, 1_c_int, mode, real(inp%tilt, c_double), seed, sweep, &
-               int(nsw_warm, c_int64_t), 1_c_int), 'main')
+    ! parity restarts at 1 (pt.F90:773-908)
+    write(out_u, '(/1x,85a)') '*------------------------------ SWAP in Monte-Carlo --------------------------------*'
+    write(out_u, '(1x,85a)') '|     Num_Steps                     Configuration Numbers                           |'
+    write(out_u, '(1x,85a)') '+-----------------------------------------------------------------------------------+'
+    write(out_u, *) 'Initial configuration = ', smap
+    call pt_rounds(nsw, steps_mc, 1_c_int, nsw_warm, 'main')
+    write(out_u, *) 'Final configuration = ', smap
+    write(out_u, '(1x,85a)') '*-----------------------------------------------------------------------------------*'
     if (inp%energy_write) call write_energy_series(steps_mc)
     call check(esp_get_stats(ctx, acc, rej, erun, nmeas), 'esp_get_stats')
     call check(esp_get_map(ctx, smap, nswap), 'esp_get_map')
     call check(esp_energy(ctx, edir), 'esp_energy')
     call check(esp_get_spins(ctx, 1_c_int, spins), 'esp_get_spins')
-    write(out_u, '(/1x,a)') '*------------------------------------ MAIN ------------------------------------------*'
-    write(out_u, '(1x,a,1000i4)') 'Final configuration = ', smap
+    write(out_u, *) ''
     call swap_table('MONTECARLO')
-    call write_mc_tables(int(2 * w1, 8) * inp%steps_measure * N, int(steps_mc, 8) * inp%steps_measure * N, .true.)
+    call write_mc_tables(0_8, int(steps_mc, 8) * inp%steps_measure * N, .true.)
   end if
 
   ! ---- _sconfig.dat (mcarlo.F90:109-113, 813-816) ----------------------------------------
-  open(newunit=sc_u, file=trim(seedname)//'_sconfig.dat', status='replace', action='write')
-  write(sc_u, '(a)') ' Created by mc_gpu.x'
-  write(sc_u, '(3I6)') inp%L
-  write(sc_u, '(2I6)') N, R
-  do k = 1, R
-    write(sc_u, '(F12.6)') tems(k)
-    do i = 1, N
-      write(sc_u, '(3F12.6)') spins(:, i, k)
-    end do
-  end do
-  close(sc_u)
+  call write_sconfig(trim(seedname)//'_sconfig.dat', .true.)
 
   call check(esp_destroy(ctx), 'esp_destroy')
   call system_clock(t1)
   write(out_u, '(/1x,a,f12.3,a)') 'Time for montecarlo ', real(t1 - t0, dp) / real(rate, dp), ' (sec)'
   close(out_u)
+224 format(3x,f9.4,1x,i14,1x,i14,5x,i14,7x,f10.8,4x,a)   ! pt.F90:1126
 
 contains
 
@@ -282,6 +347,98 @@ contains
       write(out_u, '(1x,a,i4,a,f14.6)') '   epoch ', e, '  mean -log q = ', losses(1)
     end do
     mode = ESP_MODE_LEARNED
+  end subroutine
+
+  ! A seed from /dev/urandom when the input gives none (comms_init_random, comms.F90:995-999).
+  integer function urandom_seed() result(v)
+    integer :: u, ios
+    integer(c_int64_t) :: c
+    v = 0
+    open(newunit=u, file='/dev/urandom', access='stream', form='unformatted', action='read', iostat=ios)
+    if (ios == 0) then
+      read(u, iostat=ios) v
+      close(u)
+    end if
+    if (ios /= 0) then
+      call system_clock(c)
+      v = int(mod(c, 2147483647_c_int64_t))
+    end if
+    v = abs(v)   ! the reference's MT stream equals standard MT19937 for non-negative seeds
+  end function
+
+  ! _sconfig.dat / _sconfig_warmup.dat: date line, (supercell line), N and the temperature
+  ! count, then per temperature T (F12.6) and the spins (3F12.6) (mcarlo.F90:109-114,813-816;
+  ! pt.F90:610-629).  spins must be by temperature slot.
+  subroutine write_sconfig(fname, with_cell)
+    character(len=*), intent(in) :: fname
+    logical, intent(in) :: with_cell
+    character(len=8) :: d
+    character(len=10) :: tm
+    call date_and_time(date=d, time=tm)
+    open(newunit=sc_u, file=fname, status='replace', action='write')
+    write(sc_u, *) 'Created on '//d(7:8)//'/'//d(5:6)//'/'//d(1:4)//' at '//tm(1:2)//':'//tm(3:4)//':'//tm(5:6)
+    if (with_cell) write(sc_u, '(3I6)') inp%L
+    write(sc_u, '(2I6)') N, R
+    do k = 1, R
+      write(sc_u, '(F12.6)') tems(k)
+      do i = 1, N
+        write(sc_u, '(3F12.6)') spins(:, i, k)
+      end do
+    end do
+    close(sc_u)
+  end subroutine
+
+  ! PT energy table (format 127): T, initial, final by adding, final direct, by slot.
+  subroutine energy_table(phase, ei, ea, ed, tag)
+    character(len=*), intent(in) :: phase, tag
+    real(c_double), intent(in) :: ei(:), ea(:), ed(:)
+    integer :: t
+    write(out_u, '(/1x,a)') '*------------------------------------ '//phase//' --------------------------------------*'
+    write(out_u, '(1x,a)') '|   Temp             E_init             E_final(by adding)       E_final(direct)    |'
+    write(out_u, '(1x,a)') '+-----------------------------------------------------------------------------------+'
+    do t = 1, R
+      write(out_u, 227) tems(t), ei(t), ea(t), ed(t), '  <-- '//tag
+    end do
+    write(out_u, '(1x,a)') '*---------------------------------- END of '//phase//' ---------------------------------*'
+227 format(2x,f9.4,8x,E15.8,8x,E15.8,8x,E15.8,2x,a)
+  end subroutine
+
+  ! nrounds exchange rounds (pt_steps_swap MC steps each) of the phase that ends after
+  ! nsteps MC steps, printing the replica map at the start of MC step 1, of every multiple of
+  ! pt_print_swap and of the last step (pt.F90:661-662,780-781).  Rounds between prints run
+  ! as one graph-replayed esp_run.
+  subroutine pt_rounds(nrounds, nsteps, measure, step0, what)
+    integer, intent(in) :: nrounds, nsteps, step0
+    integer(c_int), intent(in) :: measure
+    character(len=*), intent(in) :: what
+    integer :: done, n, kr, target
+    integer(8) :: p
+    done = 0
+    n = 1
+    do while (n <= nsteps)
+      kr = (n - 1) / inp%pt_steps_swap        ! exchange rounds before MC step n
+      if (kr > done) then
+        call check(esp_run(ctx, int(kr - done, c_int), int(inp%pt_steps_swap, c_int), &
+                   int(inp%steps_measure, c_int), measure, 1_c_int, int(1 + mod(done, 2), c_int), mode, &
+                   real(inp%tilt, c_double), seed, sweep + int(done, c_int64_t) * inp%pt_steps_swap * &
+                   inp%steps_measure, int(step0 + done, c_int64_t), 1_c_int), what)
+        done = kr
+        call check(esp_get_map(ctx, smap, nswap), 'esp_get_map')
+      end if
+      write(out_u, *) n, ' | ', smap
+      ! next printed step: the next multiple of pt_print_swap, or the last step
+      p = (int(n, 8) / max(1, inp%pt_print_swap) + 1) * max(1, inp%pt_print_swap)
+      if (n == nsteps) exit
+      target = int(min(p, int(nsteps, 8)))
+      n = target
+    end do
+    if (nrounds > done) then
+      call check(esp_run(ctx, int(nrounds - done, c_int), int(inp%pt_steps_swap, c_int), &
+                 int(inp%steps_measure, c_int), measure, 1_c_int, int(1 + mod(done, 2), c_int), mode, &
+                 real(inp%tilt, c_double), seed, sweep + int(done, c_int64_t) * inp%pt_steps_swap * &
+                 inp%steps_measure, int(step0 + done, c_int64_t), 1_c_int), what)
+      call check(esp_get_map(ctx, smap, nswap), 'esp_get_map')
+    end if
   end subroutine
 
   subroutine check(st, what)
@@ -412,22 +569,27 @@ contains
       dM = sqrt(max(acc(5, t) - acc(4, t)**2, 0.0_dp) * nrm)
       if (nwarm > 0) write(out_u, 124) tems(t), nwarm, rej_w(t), nwarm - rej_w(t), &
                                        real(nwarm - rej_w(t), dp) / real(nwarm, dp), '  <-- WARMUP'
-      write(out_u, 124) tems(t), nmc, rej(t), nmc - rej(t), real(nmc - rej(t), dp) / real(nmc, dp), &
-                        '  <-- MCARLO'
       if (.not. pt) then
+        write(out_u, 124) tems(t), nmc, rej(t), nmc - rej(t), real(nmc - rej(t), dp) / real(nmc, dp), &
+                          '  <-- MCARLO'
         write(out_u, '(4x,a,E15.7,a)') ' Initial Energy                           = ', e0(t), '  <-- ENERGY'
         write(out_u, '(4x,a,E15.7,a)') ' Final Energy by adding differences       = ', erun(t), '  <-- ENERGY'
         write(out_u, '(4x,a,E15.7,a)') ' Final Energy by direct calculations      = ', edir(t), '  <-- ENERGY'
-      else
+      else              ! pt.F90:1019-1026: energies of the configuration at each temperature slot
+        write(out_u, 224) tems(t), nmc, rej(t), nmc - rej(t), real(nmc - rej(t), dp) / real(nmc, dp), &
+                          '  <-- MCARLO'
+        write(out_u, '(4x,a,E15.7,a)') ' Initial Energy                           = ', e_init(t), '  <-- ENERGY3'
         write(out_u, '(4x,a,E15.7,a)') ' Final Energy by adding differences       = ', erun(smap(t)), '  <-- ENERGY3'
         write(out_u, '(4x,a,E15.7,a)') ' Final Energy by direct calculations      = ', edir(smap(t)), '  <-- ENERGY3'
       end if
       write(out_u, '(4x,a,f15.9)') '                             Delta E      = +/- ', dE
       write(out_u, '(4x,a,f15.9)') '                             Delta M      = +/- ', dM
+      if (pt) write(out_u, '(1x,a85)') repeat('-', 85)
       write(mc_u, 125) tems(t), acc(4, t), acc(1, t) / real(N, dp), C, chi, UE, UM
     end do
     close(mc_u)
 124 format(3x,f9.4,1x,i14,1x,i14,5x,i14,7x,f10.8,1x,a)
+224 format(3x,f9.4,1x,i14,1x,i14,5x,i14,7x,f10.8,4x,a)
 125 format(f9.4,2x,E15.8,2x,E15.8,2x,E15.8,2x,E15.8,2x,E15.8,2x,E15.8)
   end subroutine
 

@@ -13,7 +13,7 @@ module mc_input
   real(dp), parameter, public :: k_B = 8.6173303e-5_dp          ! constants.F90:48
   real(dp), parameter, public :: hart = 27.21138602_dp          ! constants.F90:49
   real(dp), parameter, public :: bohr_magn = 5.7883818012e-5_dp ! constants.F90:50
-  real(dp), parameter, public :: bohr = 0.52917721092_dp
+  real(dp), parameter, public :: bohr = 0.5291772108_dp       ! constants.F90:37
   integer, parameter :: maxl = 512
 
   type, public :: rows_t
@@ -47,6 +47,7 @@ module mc_input
   public :: read_input, input_error
 
   character(len=maxl), allocatable :: lines(:)
+  logical, allocatable :: used(:)      ! lines consumed by a keyword or block (param_read, parameters.F90:999-1030)
   integer :: nlines
 
 contains
@@ -80,7 +81,8 @@ contains
       if (ios /= 0) exit
       nlines = nlines + 1
     end do
-    allocate(lines(nlines))
+    allocate(lines(nlines), used(nlines))
+    used = .false.
     rewind(u)
     do k = 1, nlines
       read(u, '(a)') buf
@@ -112,6 +114,7 @@ contains
       if (rest(1:1) == '=' .or. rest(1:1) == ':') rest = adjustl(rest(2:))
       if (found) call input_error('keyword '//trim(key)//' found more than once')
       found = .true.
+      used(k) = .true.
       val = rest
     end do
   end subroutine
@@ -146,6 +149,7 @@ contains
     do k = first, nlines
       if (lines(k)(1:3) == 'end') then
         last = k - 1
+        used(first - 1:k) = .true.
         return
       end if
     end do
@@ -494,7 +498,75 @@ contains
       call read_axes(inp, 'field_axes', inp%h_par, inp%h_axis, found)
       if (.not. found) call input_error('ham_field is true but no field_axes block found')
     end if
+    call ignored_keywords()
+    call unsupported_keywords()
+    call unrecognised_keywords(fname)
   contains
+    ! Reference keywords of features this driver does not run yet: accepted when off, an
+    ! error when switched on (never silently ignored).
+    subroutine unsupported_keywords()
+      character(len=maxl) :: w
+      character(len=24), parameter :: flags(6) = [character(len=24) :: 'order_parameter', 'staggered_m', &
+                                                   'spin_correlation', 'sfactor', 'binning_error', 'spin_glass']
+      character(len=24), parameter :: values(9) = [character(len=24) :: 'staggered_m_coeff', 'sfactor_2dqmesh', &
+                                                   'sfactor_corner', 'sfactor_polar', 'sfactor_q1', 'sfactor_q2', &
+                                                   'sfactor_steps_measure', 'binning_level', 'spin_glass_seed']
+      integer :: q, f1, f2
+      logical :: fb
+      do q = 1, size(flags)
+        call keyword(trim(flags(q)), found, w)
+        if (found) then
+          if (is_true(w)) call input_error(trim(flags(q))//' = true is not supported by mc_gpu.x')
+        end if
+      end do
+      do q = 1, size(values)
+        call keyword(trim(values(q)), found, w)
+      end do
+      call block('order_parameter_axes_cart', fb, f1, f2)
+      call block('order_parameter_axes_frac', fb, f1, f2)
+    end subroutine
+
+    ! Keywords of the reference that only its input-generation modes (-inp1/-inp2/-ham) read,
+    ! or that change only how its output is buffered: accepted with their reference checks.
+    subroutine ignored_keywords()
+      character(len=maxl) :: w
+      call keyword('parameter_unit', found, w)
+      if (found .and. w(1:2) /= 'ev' .and. w(1:12) /= 'electronvolt' .and. w(1:3) /= 'ryd') &
+        call input_error('Value of parameter_unit not recognised in param_read')
+      call keyword('coordinate', found, w)
+      if (found .and. w(1:4) /= 'frac' .and. w(1:4) /= 'cart') &
+        call input_error('Value of coordinate not recognised in param_read')
+      call keyword('length_unit', found, w)
+      if (found .and. w(1:3) /= 'ang' .and. trim(w) /= 'bohr') &
+        call input_error('Value of length_unit not recognised in param_read')
+      call keyword('neighbors_tol', found, w)
+      call keyword('inputfile_setup1', found, w)
+      call keyword('inputfile_setup2', found, w)
+      call keyword('hamfile_setup', found, w)
+      call keyword('shells_jij', found, w)
+      call keyword('shells_bij', found, w)
+      call keyword('shells_dij', found, w)
+      call keyword('energy_num_print', found, w)
+      if (found) then
+        read(w, *, iostat=ios) k
+        if (ios /= 0 .or. k < 1) call input_error('energy_num_print must be greater than zero')
+      end if
+    end subroutine
+
+    ! Any non-blank line no keyword or block consumed is fatal, as in the reference
+    ! (parameters.F90:999-1030): the lines are printed, then the run stops.
+    subroutine unrecognised_keywords(fn)
+      character(len=*), intent(in) :: fn
+      integer :: q
+      if (all(used .or. len_trim(lines) == 0)) return
+      write(*, '(1x,a)') 'The following section of file '//trim(fn)//' contained unrecognised keywords'
+      write(*, *)
+      do q = 1, nlines
+        if (.not. used(q) .and. len_trim(lines(q)) > 0) write(*, '(1x,a)') trim(lines(q))
+      end do
+      write(*, *)
+      call input_error('Unrecognised keyword(s) in input file')
+    end subroutine
     subroutine sort(t)
       real(dp), intent(inout) :: t(:)
       integer :: i, j

@@ -146,6 +146,42 @@ double orc_energy(const orc_model *m, const double *s) {
   return e / 2.0;
 }
 
+/* The terms energy_get sums, grouped by site: e_site[i] = the J, field, single-ion, B and D
+ * terms of site i in the order energy_get adds them (get_quant.F90:210-257), so that
+ * energy = (sum_i e_site[i]) / 2.  Summed exactly (e.g. math.fsum) this gives the correctly
+ * rounded energy, free of the rounding of one long sequential sum. */
+void orc_energy_sites(const orc_model *m, const double *s, double *e_site) {
+  const int N = m->N;
+  for (int i = 0; i < N; ++i) {
+    double e = 0.0;
+    const int a = basis_of(m, i);
+    if (m->zJ > 0)
+      for (int l = 0; l < m->j_cnt[i]; ++l) {
+        int j = m->j_nbr[(size_t)l * N + i] - 1;
+        e = e - m->j_val[(size_t)l * N + i] * dot3(s + 3 * i, s + 3 * j);
+      }
+    if (m->have_field) e = e + 2.0 * m->h_par[a] * dot3(s + 3 * i, m->h_axis + 3 * a);
+    if (m->have_si) {
+      double sa = dot3(s + 3 * i, m->si_axis + 3 * a);
+      e = e + 2.0 * m->si_par[a] * sa * sa;
+    }
+    if (m->zB > 0)
+      for (int l = 0; l < m->b_cnt[i]; ++l) {
+        int j = m->b_nbr[(size_t)l * N + i] - 1;
+        double d = dot3(s + 3 * i, s + 3 * j);
+        e = e + m->b_val[(size_t)l * N + i] * d * d;
+      }
+    if (m->zD > 0)
+      for (int l = 0; l < m->d_cnt[i]; ++l) {
+        int j = m->d_nbr[(size_t)l * N + i] - 1;
+        double cc[3];
+        cross3(s + 3 * i, s + 3 * j, cc);
+        e = e + dot3(m->d_vec + 3 * ((size_t)l * N + i), cc);
+      }
+    e_site[i] = e;
+  }
+}
+
 /* delta_energy_get (get_quant.F90:264-337).  s holds the NEW spin at i0; the old
  * spin is recovered as s_i - ds exactly as the reference does (:306, :320). */
 double orc_delta_energy(const orc_model *m, int i0, const double *s, const double ds[3]) {

@@ -51,6 +51,9 @@ typedef struct {
 
 /* energy_get (get_quant.F90:186-261) */
 double orc_energy(const orc_model *m, const double *s);
+/* The same terms grouped by site, e_site[N]: energy = (sum_i e_site[i]) / 2 (exact summation
+ * gives the correctly rounded energy; the long sequential sum above rounds as the reference's). */
+void orc_energy_sites(const orc_model *m, const double *s, double *e_site);
 /* delta_energy_get (get_quant.F90:264-337); s already holds the proposed spin at site i0 */
 double orc_delta_energy(const orc_model *m, int i0, const double *s, const double ds[3]);
 /* magnetization_get (get_quant.F90:340-363): |m|^2 with m = sum_j S_j / N */

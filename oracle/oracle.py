@@ -43,6 +43,7 @@ for name, res, args in [
     ("orc_mt_fill_u32", None, [C.c_int32, _i64, _u32p]),
     ("orc_philox4x32_10", None, [_u32p, _u32p, _u32p]),
     ("orc_energy", _d, [_mp, _dp]),
+    ("orc_energy_sites", None, [_mp, _dp, _dp]),
     ("orc_delta_energy", _d, [_mp, _i, _dp, _dp]),
     ("orc_magnetization2", _d, [_mp, _dp]),
     ("orc_propose", None, [_i, _d, _d, _dp, _d, _d, _dp]),
@@ -100,6 +101,15 @@ class Oracle:
     def energy(self, s):
         s = np.ascontiguousarray(s, np.float64)
         return lib.orc_energy(C.byref(self.m), _p(s, _d))
+
+    def energy_exact(self, s):
+        """energy_get's terms summed exactly (math.fsum over the per-site groups): the correctly
+        rounded energy, against which long sequential sums are judged."""
+        import math
+        s = np.ascontiguousarray(s, np.float64)
+        e = np.zeros(self.N)
+        lib.orc_energy_sites(C.byref(self.m), _p(s, _d), _p(e, _d))
+        return math.fsum(e.tolist()) / 2.0
 
     def delta_energy(self, i0, s_new, ds):
         s_new = np.ascontiguousarray(s_new, np.float64)

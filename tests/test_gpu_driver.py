@@ -118,3 +118,64 @@ def test_energy_series_files(tmp_path):
         assert data.shape == (nsteps, 2)
         assert np.array_equal(data[:, 0], np.arange(1, nsteps + 1))
         assert data[:, 1].mean() == pytest.approx(mine[t, 2], rel=1e-6, abs=1e-7)
+
+
+def _rand_config(m, seed):
+    """The reference's 'rand' start (mcarlo.F90:275-286): two draws of its MT19937 stream per
+    site, S = m (sin(pi r1) cos(2 pi r2), sin(pi r1) sin(2 pi r2), cos(pi r1)); the oracle's
+    MT19937 is pinned to the reference's own mtprng.f90."""
+    from oracle import oracle as O
+    import math
+    st = O.mt_state(seed)
+    s = np.zeros((m.N, 3))
+    for i in range(m.N):
+        r1 = O.lib.orc_mt_real2(st)
+        r2 = O.lib.orc_mt_real2(st)
+        s[i] = m.mag[i] * np.array([math.sin(r1 * math.pi) * math.cos(r2 * 2 * math.pi),
+                                    math.sin(r1 * math.pi) * math.sin(r2 * 2 * math.pi),
+                                    math.cos(r1 * math.pi)])
+    return s
+
+
+def test_rand_start_is_the_reference_stream(tmp_path):
+    """initial_sconfig = rand with seeds = s: every temperature starts from the configuration
+    the reference draws from its MT19937 stream seeded with s -- the printed initial energies
+    equal the oracle energy of that configuration -- and a second run reproduces the first."""
+    from espins_amd import mcin
+    from oracle import oracle as O
+    edits = {"steps_warmup": 100, "steps_mc": 200, "seeds": 4357, "initial_sconfig": "rand"}
+    _prepare(tmp_path, "ex01_cubic.mcin", "cubic", edits)
+    out = _run(tmp_path, "cubic")
+    init = [float(x) for x in re.findall(r"Initial Energy\s*=\s*(\S+)", out)]
+    m = mcin.to_model(mcin.read(str(tmp_path / "cubic.mcin")), builder=O.build_table_fn)
+    e_ref = O.Oracle(m).energy(_rand_config(m, 4357))
+    assert len(init) == 20
+    assert np.allclose(init, e_ref, rtol=1e-6, atol=1e-6), (init[:2], e_ref)
+    first = (tmp_path / "cubic_mc.dat").read_text().splitlines()[1:]
+    _run(tmp_path, "cubic")
+    assert (tmp_path / "cubic_mc.dat").read_text().splitlines()[1:] == first
+
+
+def test_pt_warmup_outputs_and_map_prints(tmp_path):
+    """Parallel tempering: the WARMUP1 / WARMUP2 tables, _sconfig_warmup.dat after warmup1, and
+    the replica map printed at step 1, every pt_print_swap steps and at the last step of each
+    phase (pt.F90:557-713, 773-781)."""
+    _prepare(tmp_path, "ex02_cubic.mcin", "cubic",
+             {"steps_warmup": 400, "steps_mc": 600, "supercell_size": "4 4 4", "seeds": 11,
+              "energy_write": ".false.", "pt_print_swap": 150})
+    out = _run(tmp_path, "cubic")
+    for tag in ("<-- WARMUP1", "<-- ENERGY1", "<-- WARMUP2", "<-- ENERGY2", "<-- MCARLO"):
+        assert out.count(tag) == 32, tag
+    assert out.count("<-- ENERGY3") == 3 * 32          # initial, by adding, direct
+    # warm-up halves: 200 plain + 200 with exchanges (pt.F90:529-537); main 600 steps
+    prints = [int(l.split("|")[0]) for l in out.splitlines() if re.match(r"^\s*\d+\s+\|", l)]
+    assert prints == [1, 150, 200] + [1, 150, 300, 450, 600]
+    lines = (tmp_path / "cubic_sconfig_warmup.dat").read_text().splitlines()
+    assert lines[1].split() == ["64", "32"]
+    s = np.array([[float(v) for v in l.split()] for l in lines[3:67]])
+    assert np.allclose(np.linalg.norm(s, axis=1), 1.0, atol=2e-6)
+    # running and direct energies agree at the end of both exchange phases
+    for tag in ("ENERGY2",):
+        rows = [l.split() for l in out.splitlines() if l.rstrip().endswith("<-- " + tag)]
+        e = np.array([[float(r[2]), float(r[3])] for r in rows])
+        assert np.allclose(e[:, 0], e[:, 1], rtol=1e-6)

@@ -258,3 +258,65 @@ def test_lds_mid_launch_measurement_matches_global_path(En, steps, M):
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1], outs[1][1])
     np.testing.assert_allclose(outs[0][2], outs[1][2], rtol=1e-13)
+
+
+@pytest.mark.parametrize("name", ["C2_fcc16_j123_si", "C4_sc64_jbd_si", "C5_fcc32_jb_si"])
+def test_full_size_energies_match_oracle(En, name):
+    """The benchmark Hamiltonians at their full sizes: direct energies of random and ferro
+    configurations equal the oracle's energy_get restatement to 1e-12 relative."""
+    from oracle import oracle as Oc
+    if name.startswith("C2"):
+        m, _ = Lt.config_c2(L=16, R=2)
+    elif name.startswith("C4"):
+        m, _ = Lt.config_c4(L=64, R=2)
+    else:
+        m, _ = Lt.config_c5(L=32, R=2)
+    say("full-size energy", name, m.N)
+    spins = Lt.random_spins(m.N, 3, 17, m.mag)
+    spins[2] = Lt.ferro_spins(m.N, 1, m.mag)[0]
+    with En.Engine() as eng:
+        eng.set_model(m).set_replicas(np.array([0.5, 1.0, 2.0]), spins)
+        E = eng.energy_get()
+    o = Oc.Oracle(m)
+    for r in range(3):
+        # against the exactly summed energy_get terms: 1e-12 relative
+        Ex = o.energy_exact(spins[r])
+        assert abs(E[r] - Ex) <= 1e-12 * abs(Ex), (r, E[r], Ex)
+        # energy_get's own sequential sum differs from it by at most its rounding (N eps)
+        Eo = o.energy(spins[r])
+        assert abs(E[r] - Eo) <= 64 * m.N * 2.3e-16 * abs(Eo), (r, E[r], Eo)
+
+
+def test_batched_chain_matches_oracle_philox_sweep(En):
+    """A chain of a batch advances exactly like the oracle's colour-order Philox sweep with the
+    chain's stream base k << 16 (replica r draws stream (k << 16) + r)."""
+    from espins_amd.engine import plan_model
+    chains = [(Lt.cubic_model("fcc", 4, J=(1.0, 0.15)), np.array([0.6, 1.2, 2.0])),
+              (Lt.cubic_model("sc", 8, J=(1.0,), B=0.05, D=0.3, K=0.02), np.array([0.4, 0.9])),
+              (Lt.cubic_model("bcc", 6, J=(1.0, -0.2)), np.array([0.7, 1.4, 2.1, 3.0]))]
+    init = [Lt.random_spins(m.N, len(t), 40 + k, m.mag) for k, (m, t) in enumerate(chains)]
+    seed, nsw = 0xABCDEF12345, 3
+    with En.Batch() as b:
+        for (m, t), s in zip(chains, init):
+            b.add_chain(m, t, s)
+        for k in range(len(chains)):
+            b.set_chain_config(k, 1, 1)          # global-memory colour phases
+        b.commit()
+        b.run(nsw, measure=False, exchange=False, seed=seed, graph=True)
+        got = [b.spins(k) for k in range(len(chains))]
+        rej = [b.stats(k)["rejected"] for k in range(len(chains))]
+    for k, ((m, t), s0) in enumerate(zip(chains, init)):
+        ncol, col, _ = plan_model(m)
+        sites = np.argsort(col, kind="stable").astype(np.int32)
+        off = np.concatenate([[0], np.cumsum(np.bincount(col, minlength=ncol))]).astype(np.int32)
+        o = O.Oracle(m)
+        for r in range(len(t)):
+            s = s0[r].copy()
+            e = o.energy(s)
+            nr = 0
+            for sw in range(nsw):
+                e, x = o.sweep_colour_philox(s, off, sites, 1.0 / t[r], 0, 0.125, seed, sw,
+                                             (k << 16) + r, e)
+                nr += x
+            assert rej[k][r] == nr, (k, r)
+            np.testing.assert_allclose(got[k][r], s, rtol=0, atol=1e-12)

@@ -439,3 +439,26 @@ def test_colouring_check_ignores_self_bonds():
     m = Lt.Model("sc", (1, 1, 2), 1, np.zeros((1, 3)), np.ones(2), t)
     assert O.Oracle(m).check_colouring(np.array([0, 1], np.int32))
     assert not O.Oracle(m).check_colouring(np.array([0, 0], np.int32))
+
+
+def test_exactly_summed_energy_pins():
+    """orc_energy_sites + fsum: the energy_get terms summed without accumulated rounding.  With
+    couplings that are exact binary fractions the ferro energy is exact in closed form and both
+    sums hit it; with 0.1-type couplings (N = 131072) the exactly summed energy stays within a
+    few ulp of the closed form while energy_get's long sequential sum drifts by its rounding."""
+    for J, B, Ksi in (((-1.0, 0.125), 0.125, -0.0625), ((-1.0, 0.1), 0.1, -0.05)):
+        m = Lt.cubic_model("fcc", 32, J=J, B=B, K=Ksi, builder=O.build_table_fn)
+        o = O.Oracle(m)
+        s = Lt.ferro_spins(m.N, 1)[0]
+        want = _closed_ferro("fcc", 32, J, B, Ksi)
+        if J[1] == 0.125:
+            assert o.energy_exact(s) == want and o.energy(s) == want
+        else:
+            assert abs(o.energy_exact(s) - want) <= 1e-14 * abs(want)
+            assert abs(o.energy(s) - want) < 64 * m.N * 2.3e-16 * abs(want)   # ~31 N terms added in turn
+        r = Lt.random_spins(m.N, 1, 3)[0]
+        assert o.energy_exact(r) == pytest.approx(o.energy(r), rel=64 * m.N * 2.3e-16, abs=1e-9)
+    # a small case with every term: the grouping by site sums the same terms as energy_get
+    ms = Lt.cubic_model("sc", 3, J=(1.0, 0.3), B=0.05, D=0.3, K=0.02, field=0.5, builder=O.build_table_fn)
+    rs = Lt.random_spins(ms.N, 1, 4)[0]
+    assert O.Oracle(ms).energy_exact(rs) == pytest.approx(O.Oracle(ms).energy(rs), rel=1e-14)
