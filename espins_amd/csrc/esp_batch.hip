@@ -28,15 +28,15 @@ namespace {
 // ---- batched kernels -------------------------------------------------------------------
 // tiles: (chain, replica, block, 0); chain < 0 = padding tile (XCD interleave)
 template <int MODE, int LAY, bool HB, bool HD, bool SELF>
-__global__ __launch_bounds__(kBlock) void k_bphase(const KArgs *chains, const int4 *tiles, int colour,
+__global__ __launch_bounds__(kBlock) ESP_PHASE_ATTR void k_bphase(const KArgs *chains, const int4 *tiles, int colour,
                                                    int sweep_off) {
   extern __shared__ double s_ctab[];
   const int4 t = tiles[blockIdx.x];
   if (t.x < 0) return;
   KArgs a = chains[t.x];
   a.col = colour;
-  a.c_begin = a.col_off[colour];
-  a.c_count = a.col_off[colour + 1] - a.c_begin;
+  a.c_begin = gldi(a.col_off, colour);
+  a.c_count = gldi(a.col_off, colour + 1) - a.c_begin;
   a.sweep_off = sweep_off;
   phase_body<MODE, LAY, HB, HD, SELF>(a, t.y, t.z, s_ctab);
 }
@@ -364,8 +364,21 @@ int build_plan(esp_batch *b, double tilt, uint64_t seed) {
         if (c.variant != v.variant || c.lds || col >= c.hm.ncol) continue;
         const int cnt = c.hm.col_off[col + 1] - c.hm.col_off[col];
         const int nblk = (cnt * c.tps + kBlock - 1) / kBlock;
+        // Implicit geometry: every group of the colour has the same sub-cell layout, so the
+        // blocks go in sub-cell order across the groups (block (g, t) of every group g before
+        // any block t + 1).  Blocks that run together then gather from the same region of the
+        // lattice and share its lines in L2 (about 30 % fewer L2 misses than group order).
+        std::vector<int> order(nblk);
+        for (int bb = 0; bb < nblk; ++bb) order[bb] = bb;
+        if (c.hm.implicit) {
+          const int per = kBlock / c.tps, mt = (int)c.hm.geo.mt.d;
+          std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            const int ox = x * per, oy = y * per;
+            return std::make_pair(ox % mt, ox / mt) < std::make_pair(oy % mt, oy / mt);
+          });
+        }
         for (int r = 0; r < c.R; ++r, ++qi)
-          for (int bb = 0; bb < nblk; ++bb) q[qi % 8].push_back(make_int4(k, r, bb, 0));
+          for (int bb = 0; bb < nblk; ++bb) q[qi % 8].push_back(make_int4(k, r, order[bb], 0));
       }
       size_t longest = 0;
       for (auto &x : q) longest = std::max(longest, x.size());
@@ -640,11 +653,11 @@ int esp_batch_commit(esp_batch *b) {
     if ((rc = balloc(b, c.erun, R))) return rc;
     if ((rc = balloc(b, c.acc, (size_t)6 * R))) return rc;
     if ((rc = balloc(b, c.partial, (size_t)4 * R * c.red_blocks))) return rc;
-    if ((rc = balloc(b, c.rej, R))) return rc;
+    if ((rc = balloc(b, c.rej, (size_t)kRejSub * R))) return rc;
     if ((rc = balloc(b, c.nswap, (size_t)2 * R))) return rc;
     if ((rc = balloc(b, c.nmeas, 1))) return rc;
     BCHK(b, hipMemsetAsync(c.acc, 0, sizeof(double) * 6 * R, b->stream));
-    BCHK(b, hipMemsetAsync(c.rej, 0, sizeof(unsigned long long) * R, b->stream));
+    BCHK(b, hipMemsetAsync(c.rej, 0, sizeof(unsigned long long) * kRejSub * R, b->stream));
     BCHK(b, hipMemsetAsync(c.nswap, 0, sizeof(unsigned long long) * 2 * R, b->stream));
     BCHK(b, hipMemsetAsync(c.nmeas, 0, sizeof(unsigned long long), b->stream));
     double *d_in = nullptr;
@@ -1089,9 +1102,12 @@ int esp_batch_get_stats(esp_batch *b, int k, double *acc, int64_t *rejected, dou
     BCHK(b, hipMemcpyAsync(e_run, c.erun, sizeof(double) * c.R, hipMemcpyDeviceToHost, b->stream));
   }
   if (acc) BCHK(b, hipMemcpyAsync(acc, c.acc, sizeof(double) * 6 * c.R, hipMemcpyDeviceToHost, b->stream));
-  if (rejected) BCHK(b, hipMemcpyAsync(rejected, c.rej, sizeof(int64_t) * c.R, hipMemcpyDeviceToHost, b->stream));
+  std::vector<int64_t> rsub(rejected ? (size_t)kRejSub * c.R : 0);
+  if (rejected)
+    BCHK(b, hipMemcpyAsync(rsub.data(), c.rej, sizeof(int64_t) * rsub.size(), hipMemcpyDeviceToHost, b->stream));
   if (nmeas) BCHK(b, hipMemcpyAsync(nmeas, c.nmeas, sizeof(int64_t), hipMemcpyDeviceToHost, b->stream));
   BCHK(b, hipStreamSynchronize(b->stream));
+  if (rejected) sum_rej_sub(rsub, c.R, rejected);
   return ESP_OK;
 }
 
@@ -1147,7 +1163,7 @@ int esp_batch_reset_stats(esp_batch *b) {
   BCHK(b, hipSetDevice(b->device));
   for (auto &c : b->chains) {
     BCHK(b, hipMemsetAsync(c.acc, 0, sizeof(double) * 6 * c.R, b->stream));
-    BCHK(b, hipMemsetAsync(c.rej, 0, sizeof(unsigned long long) * c.R, b->stream));
+    BCHK(b, hipMemsetAsync(c.rej, 0, sizeof(unsigned long long) * kRejSub * c.R, b->stream));
     BCHK(b, hipMemsetAsync(c.nswap, 0, sizeof(unsigned long long) * 2 * c.R, b->stream));
     BCHK(b, hipMemsetAsync(c.nmeas, 0, sizeof(unsigned long long), b->stream));
   }

@@ -594,7 +594,7 @@ int esp_set_replicas(esp_ctx *c, int R, const double *tems, const double *spins)
   if ((rc = dalloc(c, c->d_partial, (size_t)4 * R * c->red_blocks))) return rc;
   if ((rc = dalloc(c, c->d_slot_rep, R))) return rc;
   if ((rc = dalloc(c, c->d_rep_slot, R))) return rc;
-  if ((rc = dalloc(c, c->d_rej, R))) return rc;
+  if ((rc = dalloc(c, c->d_rej, (size_t)kRejSub * R))) return rc;
   if ((rc = dalloc(c, c->d_nswap, (size_t)2 * R))) return rc;
   if ((rc = dalloc(c, c->d_nmeas, 1))) return rc;
   std::vector<double> beta(R);
@@ -605,7 +605,7 @@ int esp_set_replicas(esp_ctx *c, int R, const double *tems, const double *spins)
   HIPCHK(c, hipMemcpyAsync(c->d_slot_rep, id.data(), sizeof(int) * R, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_rep_slot, id.data(), sizeof(int) * R, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemsetAsync(c->d_acc, 0, sizeof(double) * 6 * R, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->d_rej, 0, sizeof(unsigned long long) * R, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->d_rej, 0, sizeof(unsigned long long) * kRejSub * R, c->stream));
   HIPCHK(c, hipMemsetAsync(c->d_nswap, 0, sizeof(unsigned long long) * 2 * R, c->stream));
   HIPCHK(c, hipMemsetAsync(c->d_nmeas, 0, sizeof(unsigned long long), c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -872,10 +872,12 @@ int esp_get_stats(esp_ctx *c, double *acc, int64_t *rejected, double *e_run, int
     HIPCHK(c, hipMemcpyAsync(e_run, c->d_erun, sizeof(double) * R, hipMemcpyDeviceToHost, c->stream));
   }
   if (acc) HIPCHK(c, hipMemcpyAsync(acc, c->d_acc, sizeof(double) * 6 * R, hipMemcpyDeviceToHost, c->stream));
+  std::vector<int64_t> rsub(rejected ? (size_t)kRejSub * R : 0);
   if (rejected)
-    HIPCHK(c, hipMemcpyAsync(rejected, c->d_rej, sizeof(int64_t) * R, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(rsub.data(), c->d_rej, sizeof(int64_t) * rsub.size(), hipMemcpyDeviceToHost, c->stream));
   if (nmeas) HIPCHK(c, hipMemcpyAsync(nmeas, c->d_nmeas, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (rejected) sum_rej_sub(rsub, R, rejected);
   return ESP_OK;
 }
 
@@ -930,7 +932,7 @@ int esp_reset_stats(esp_ctx *c) {
   if (rc) return rc;
   const int R = c->R;
   HIPCHK(c, hipMemsetAsync(c->d_acc, 0, sizeof(double) * 6 * R, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->d_rej, 0, sizeof(unsigned long long) * R, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->d_rej, 0, sizeof(unsigned long long) * kRejSub * R, c->stream));
   HIPCHK(c, hipMemsetAsync(c->d_nswap, 0, sizeof(unsigned long long) * 2 * R, c->stream));
   HIPCHK(c, hipMemsetAsync(c->d_nmeas, 0, sizeof(unsigned long long), c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));

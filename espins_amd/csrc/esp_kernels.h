@@ -11,6 +11,9 @@
 // Bonds: nbr[zmax][N] device partner indices (self-padded); coefficients per (basis, slot)
 // in LDS when the lattice is periodic, else cJ/cB[zmax][N], cD[zmax][N][3].
 #pragma once
+#include <cstdint>
+#include <vector>
+
 #include "esp_device.h"
 #include "esp_geo.h"
 
@@ -27,8 +30,31 @@ using namespace espd;
 
 constexpr int kBlock = 256;      // phase / reduce kernels (4 waves)
 constexpr int kLdsBlock = 512;   // LDS-resident kernel (8 waves per replica)
-constexpr int kChunk = 8;        // partner gathers issued together per lane
-constexpr int kChunkWide = 16;   // ... for launches too small to fill the GPU with waves
+// Tuning constants (compile-time; the defaults are the measured best on MI355X).  Fewer
+// gathers in flight per lane means fewer registers and more waves per SIMD, which hides the
+// gather latency better than a wide chunk does.
+#ifndef ESP_CHUNK
+#define ESP_CHUNK 2
+#endif
+#ifndef ESP_CHUNK_WIDE
+#define ESP_CHUNK_WIDE 4
+#endif
+#ifndef ESP_PHASE_WAVES
+#define ESP_PHASE_WAVES 0        // > 0: ask the compiler for this many waves per SIMD
+#endif
+#if ESP_PHASE_WAVES > 0
+#define ESP_PHASE_ATTR __attribute__((amdgpu_waves_per_eu(ESP_PHASE_WAVES)))
+#else
+#define ESP_PHASE_ATTR
+#endif
+constexpr int kChunk = ESP_CHUNK;  // partner gathers issued together per lane
+constexpr int kChunkWide = ESP_CHUNK_WIDE;  // ... for launches too small to fill the GPU with waves
+constexpr int kChunkLds = 8;     // ... in the LDS-resident sweep and the learned sweep
+// Rejections are counted per slot in kRejSub sub-counters, rej[slot][kRejSub]: a wave adds
+// to the sub-counter its global wave index selects, so the thousands of waves of one replica
+// do not queue at one address (a single counter per slot cost 12-14 % of a colour phase).
+// Readers sum the sub-counters.
+constexpr int kRejSub = 16;
 // Dynamic LDS of the LDS-resident kernel: [N+1 spins as double4][coefficient table]
 // [kLdsRed doubles of reduction scratch]; every byte of LDS it uses is in that one region.
 constexpr int kLdsRed = 4 * (512 / 64);
@@ -70,12 +96,31 @@ struct KArgs {
 
 enum { M_RANDOM = 0, M_CONST = 1, M_HOSTFED = 2 };
 
+// Global-memory access through address space 1.  A pointer that a kernel reads out of a
+// struct in memory (the batched kernels' per-chain arguments) is a generic pointer to the
+// compiler, which then emits flat instructions; a flat load also counts against the LDS wait
+// counter, so every LDS read after it waits for all gathers in flight.  Accesses through these
+// helpers are global loads, stores and atomics.
+#define ESP_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ T gldi(const T *p, size_t i) {
+  return ((const ESP_GLOBAL T *)p)[i];
+}
+template <class T>
+__device__ __forceinline__ void gsti(T *p, size_t i, T v) {
+  ((ESP_GLOBAL T *)p)[i] = v;
+}
+template <class T>
+__device__ __forceinline__ void gadd(T *p, T v) {     // no-return atomic add, device scope
+  __hip_atomic_fetch_add((ESP_GLOBAL T *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // 32-bit byte offset from a wave-uniform base: lets the compiler use the SGPR-base
 // addressing form (one VGPR offset per lane, no 64-bit address arithmetic).  The host keeps
 // every table and replica below 4 GiB.
 template <class T>
 __device__ __forceinline__ T ld_off(const T *base, unsigned byte_off) {
-  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+  return *(const ESP_GLOBAL T *)((const ESP_GLOBAL char *)base + byte_off);
 }
 
 struct GSpins {                  // one replica in global memory
@@ -83,6 +128,9 @@ struct GSpins {                  // one replica in global memory
   __device__ __forceinline__ V3 ld(unsigned j) const {
     const unsigned o = j * 8u;
     return {ld_off(x, o), ld_off(y, o), ld_off(z, o)};
+  }
+  __device__ __forceinline__ void st(unsigned j, const V3 &v) const {
+    gsti(x, j, v.x); gsti(y, j, v.y); gsti(z, j, v.z);
   }
 };
 struct LSpins {                  // one replica in LDS (x, y, z, w)
@@ -113,11 +161,11 @@ __device__ __forceinline__ Site site_info(const KArgs &a, int d, const int *s_gr
     s.sg = espg::geo_site(a.geo, s_grp - g0, d);
     s.ir = s.sg.ir;
     s.a = s.sg.a;
-    s.mag = a.magb[s.a];
+    s.mag = gldi(a.magb, s.a);
   } else {
-    s.ir = a.dev2ref[d];
-    s.a = a.basis[d];
-    s.mag = a.mag[d];
+    s.ir = gldi(a.dev2ref, d);
+    s.a = gldi(a.basis, d);
+    s.mag = gldi(a.mag, d);
   }
   return s;
 }
@@ -130,9 +178,9 @@ __device__ __forceinline__ Proposal propose(const KArgs &a, const SP &sp, const 
   p.so = sp.ld(st.d);
   const int ir = st.ir;
   if (MODE == M_HOSTFED) {
-    const double *q = a.prop + 3 * ((size_t)r * a.N + ir);
-    p.sn = {q[0], q[1], q[2]};
-    p.u3 = a.unif[(size_t)r * a.N + ir];
+    const size_t q = 3 * ((size_t)r * a.N + ir);
+    p.sn = {gldi(a.prop, q), gldi(a.prop, q + 1), gldi(a.prop, q + 2)};
+    p.u3 = gldi(a.unif, (size_t)r * a.N + ir);
   } else {
     const uint4 rn = philox10(make_uint4((uint32_t)ir, (uint32_t)sweep, (uint32_t)(sweep >> 32),
                                          a.stream_base + (uint32_t)r),
@@ -155,11 +203,11 @@ __device__ __forceinline__ void bond_term(const KArgs &a, const double *crow, in
     if (HB) cb = crow[5 * l + 1];
     if (HD) { cx = crow[5 * l + 2]; cy = crow[5 * l + 3]; cz = crow[5 * l + 4]; }
   } else {
-    cj = a.cJ[(size_t)l * a.N + d];
-    if (HB) cb = a.cB[(size_t)l * a.N + d];
+    cj = gldi(a.cJ, (size_t)l * a.N + d);
+    if (HB) cb = gldi(a.cB, (size_t)l * a.N + d);
     if (HD) {
-      const double *dv = a.cD + 3 * ((size_t)l * a.N + d);
-      cx = dv[0]; cy = dv[1]; cz = dv[2];
+      const size_t q = 3 * ((size_t)l * a.N + d);
+      cx = gldi(a.cD, q); cy = gldi(a.cD, q + 1); cz = gldi(a.cD, q + 2);
     }
   }
   const double dsj = dot(ds, sj);
@@ -240,6 +288,11 @@ __device__ __forceinline__ double group_sum(double v, int tps) {
   return v;
 }
 
+__device__ __forceinline__ void rej_add(unsigned long long *rej, int slot, unsigned long long n) {
+  const unsigned wv = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (kRejSub - 1);
+  gadd(&rej[(size_t)slot * kRejSub + wv], n);
+}
+
 // Local terms and the accept test of metropolis (get_quant.F90:166-178): reject iff
 // dE > 0 and exp(-beta dE) < u3.  Returns dE through *de.
 __device__ __forceinline__ bool decide(const KArgs &a, int b, const Proposal &p, double eJ,
@@ -248,14 +301,18 @@ __device__ __forceinline__ bool decide(const KArgs &a, int b, const Proposal &p,
   const V3 si = sub(p.sn, ds);
   double de = eJ;
   if (a.has_h) {
-    const double *h = a.h4 + 4 * b;
-    de = de + h[0] * (ds.x * h[1] + ds.y * h[2] + ds.z * h[3]);
+    const size_t q = 4 * (size_t)b;
+    const double h0 = gldi(a.h4, q), h1 = gldi(a.h4, q + 1), h2 = gldi(a.h4, q + 2),
+                 h3 = gldi(a.h4, q + 3);
+    de = de + h0 * (ds.x * h1 + ds.y * h2 + ds.z * h3);
   }
   if (a.has_si) {
-    const double *k = a.si4 + 4 * b;
-    const double sa = si.x * k[1] + si.y * k[2] + si.z * k[3];
-    const double dsa = ds.x * k[1] + ds.y * k[2] + ds.z * k[3];
-    de = de + k[0] * (dsa * dsa + 2.0 * sa * dsa);
+    const size_t q = 4 * (size_t)b;
+    const double k0 = gldi(a.si4, q), k1 = gldi(a.si4, q + 1), k2 = gldi(a.si4, q + 2),
+                 k3 = gldi(a.si4, q + 3);
+    const double sa = si.x * k1 + si.y * k2 + si.z * k3;
+    const double dsa = ds.x * k1 + ds.y * k2 + ds.z * k3;
+    de = de + k0 * (dsa * dsa + 2.0 * sa * dsa);
   }
   de = de + eB;
   de = de + eD;
@@ -275,6 +332,30 @@ __device__ __forceinline__ void decode_block(const KArgs &a, int &r, int &b) {
   }
 }
 
+// Sum over the 64 lanes of a wave with DPP lane moves (no LDS traffic): pairs, quads, half
+// rows, rows, then rows 0+1 into row 1 and rows 0..2 into row 3; lane 63 ends with the total,
+// which readlane makes wave-uniform.  Every lane must be active.  The order of the additions
+// is fixed, so the result is reproducible.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int2 h = __builtin_bit_cast(int2, v);
+  int2 r;
+  r.x = __builtin_amdgcn_update_dpp(0, h.x, CTRL, ROW_MASK, 0xf, false);
+  r.y = __builtin_amdgcn_update_dpp(0, h.y, CTRL, ROW_MASK, 0xf, false);
+  return __builtin_bit_cast(double, r);
+}
+__device__ __forceinline__ double wave_total(double v) {
+  v += dpp_f64<0xb1, 0xf>(v);    // quad_perm [1,0,3,2]
+  v += dpp_f64<0x4e, 0xf>(v);    // quad_perm [2,3,0,1]
+  v += dpp_f64<0x141, 0xf>(v);   // row_half_mirror
+  v += dpp_f64<0x140, 0xf>(v);   // row_mirror
+  v += dpp_f64<0x142, 0xa>(v);   // row_bcast:15 into rows 1, 3
+  v += dpp_f64<0x143, 0xc>(v);   // row_bcast:31 into rows 2, 3
+  const int2 h = __builtin_bit_cast(int2, v);
+  return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_readlane(h.x, 63),
+                                              __builtin_amdgcn_readlane(h.y, 63)));
+}
+
 // Per-wave sums of one colour phase: the accepted dE (added to the wave's accumulator, which
 // the running energy sums) and the spins after the phase (stored; a site changes only in its
 // own colour phase, so after a sweep these add up to the replica's magnetisation and the
@@ -286,22 +367,19 @@ __device__ __forceinline__ void wave_sums(const KArgs &a, int r, int b, bool own
   double w = owner && !rej ? de : 0.0;
   V3 sf = {0.0, 0.0, 0.0};
   if (owner) sf = rej ? p.so : p.sn;
-  for (int o = 1; o < 64; o <<= 1) {
-    w += __shfl_xor(w, o, 64);
-    if (a.mpart) {
-      sf.x += __shfl_xor(sf.x, o, 64);
-      sf.y += __shfl_xor(sf.y, o, 64);
-      sf.z += __shfl_xor(sf.z, o, 64);
-    }
+  w = wave_total(w);
+  if (a.mpart) {
+    sf.x = wave_total(sf.x);
+    sf.y = wave_total(sf.y);
+    sf.z = wave_total(sf.z);
   }
   // waves past the colour's last lane own no entry (mpart_nw counts the colour's waves)
   const int wave = (b * kBlock + threadIdx.x) >> 6;
   if ((threadIdx.x & 63) == 0 && wave * 64 < a.c_count * tps) {
     const size_t e = ((size_t)r * a.ncol + a.col) * a.mpart_nw + wave;
-    if (w != 0.0) atomicAdd(a.w + e, w);
+    if (w != 0.0) gadd(a.w + e, w);
     if (a.mpart) {
-      double *o = a.mpart + e * 3;
-      o[0] = sf.x; o[1] = sf.y; o[2] = sf.z;
+      gsti(a.mpart, e * 3, sf.x); gsti(a.mpart, e * 3 + 1, sf.y); gsti(a.mpart, e * 3 + 2, sf.z);
     }
   }
 }
@@ -328,7 +406,7 @@ __device__ __forceinline__ void phase_body(const KArgs &a, int r, int b, double 
   int *s_gs = nullptr, *s_grp = nullptr;
   if (CMP) {
     const int n = a.nb * a.zmax * 5;
-    for (int k = threadIdx.x; k < n; k += kBlock) s_ctab[k] = a.ctab[k];
+    for (int k = threadIdx.x; k < n; k += kBlock) s_ctab[k] = gldi(a.ctab, k);
   }
   if (IMP) {   // (group, slot) rows and group records of the groups this block's sites lie in
     const int first = a.c_begin + b * per;
@@ -340,17 +418,17 @@ __device__ __forceinline__ void phase_body(const KArgs &a, int r, int b, double 
     s_grp = s_gs + 2 * a.rows_max * a.zmax;
     const int n = 2 * nrow * a.zmax;
     const int *src = a.gslot + (size_t)2 * g0 * a.zmax;
-    for (int k = threadIdx.x; k < n; k += kBlock) s_gs[k] = src[k];
-    for (int k = threadIdx.x; k < nrow; k += kBlock) s_grp[k] = a.grp[g0 + k];
+    for (int k = threadIdx.x; k < n; k += kBlock) s_gs[k] = gldi(src, k);
+    for (int k = threadIdx.x; k < nrow; k += kBlock) s_grp[k] = gldi(a.grp, g0 + k);
   }
   if (CMP) __syncthreads();
   const int g = threadIdx.x & (tps - 1);
   const int k = (b * kBlock + threadIdx.x) / tps;
   const bool active = k < a.c_count;
   const int d = a.c_begin + (active ? k : 0);
-  const int slot = a.rep_slot[r];
-  const double beta = a.beta_slot[slot];
-  const unsigned long long sweep = a.clock->sweep + (unsigned long long)a.sweep_off;
+  const int slot = gldi(a.rep_slot, r);
+  const double beta = gldi(a.beta_slot, slot);
+  const unsigned long long sweep = gldi(&a.clock->sweep, 0) + (unsigned long long)a.sweep_off;
   double *base = a.sp + (size_t)r * 3 * a.Ns;
   const GSpins sp{base, base + a.Ns, base + 2 * a.Ns};
   // inactive lanes of a block sit on its first site, which lies in row g0
@@ -367,18 +445,18 @@ __device__ __forceinline__ void phase_body(const KArgs &a, int r, int b, double 
   if (active) {
     rej = decide(a, st.a, p, eJ, eB, eD, beta, &de);
     if (g == 0) {
-      if (MODE == M_HOSTFED) a.accepted[(size_t)r * a.N + st.ir] = rej ? 0 : 1;
-      if (!rej) { sp.x[d] = p.sn.x; sp.y[d] = p.sn.y; sp.z[d] = p.sn.z; }
+      if (MODE == M_HOSTFED) gsti(a.accepted, (size_t)r * a.N + st.ir, (uint8_t)(rej ? 0 : 1));
+      if (!rej) sp.st(d, p.sn);
     }
   }
   const unsigned long long m = __ballot(rej && g == 0);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(&a.rej[slot], (unsigned long long)__popcll(m));
+  if ((threadIdx.x & 63) == 0 && m) rej_add(a.rej, slot, (unsigned long long)__popcll(m));
   wave_sums(a, r, b, active && g == 0, rej, de, p, tps);
 }
 
 // One colour phase over every replica of one context.
 template <int MODE, int LAY, bool HB, bool HD, bool SELF, int CH>
-__global__ __launch_bounds__(kBlock) void k_phase(KArgs a) {
+__global__ __launch_bounds__(kBlock) ESP_PHASE_ATTR void k_phase(KArgs a) {
   extern __shared__ double s_ctab[];
   int r, b;
   decode_block(a, r, b);
@@ -404,22 +482,22 @@ __device__ __forceinline__ void lds_body(const KArgs &a, int r, int nsweeps, con
   const double *gx = a.sp + (size_t)r * 3 * a.Ns;
   // w of a site in LDS: its accepted dE during this launch
   for (int d = threadIdx.x; d <= a.N; d += kLdsBlock)   // index N: the zero spin
-    s_sp[d] = d < a.N ? make_double4(gx[d], gx[a.Ns + d], gx[2 * a.Ns + d], 0.0)
+    s_sp[d] = d < a.N ? make_double4(gldi(gx, d), gldi(gx, a.Ns + d), gldi(gx, 2 * (size_t)a.Ns + d), 0.0)
                       : make_double4(0.0, 0.0, 0.0, 0.0);
   if (CMP)
-    for (int k = threadIdx.x; k < a.nb * a.zmax * 5; k += kLdsBlock) ctab[k] = a.ctab[k];
+    for (int k = threadIdx.x; k < a.nb * a.zmax * 5; k += kLdsBlock) ctab[k] = gldi(a.ctab, k);
   // accepted dE the global-memory phases accumulated since the last rebase
   double wg[1] = {0.0};
   {
     const int ne = a.ncol * a.mpart_nw;
     const double *wr = a.w + (size_t)r * ne;
-    for (int k = threadIdx.x; k < ne; k += kLdsBlock) wg[0] += wr[k];
+    for (int k = threadIdx.x; k < ne; k += kLdsBlock) wg[0] += gldi(wr, k);
   }
   block_sum<1>(wg, red);     // (its barriers also order the LDS fills above)
   const LSpins sp{s_sp};
-  const int slot = a.rep_slot[r];
-  const double beta = a.beta_slot[slot];
-  const unsigned long long sweep0 = a.clock->sweep;
+  const int slot = gldi(a.rep_slot, r);
+  const double beta = gldi(a.beta_slot, slot);
+  const unsigned long long sweep0 = gldi(&a.clock->sweep, 0);
   const int tps = a.tps;
   const int g = threadIdx.x & (tps - 1);
   const int per = kLdsBlock / tps;
@@ -431,13 +509,13 @@ __device__ __forceinline__ void lds_body(const KArgs &a, int r, int nsweeps, con
   double e_last = 0.0;
   unsigned nmeas_local = 0;
   if (threadIdx.x == 0 && m.measure_every > 0)
-    for (int k = 0; k < 6; ++k) A[k] = m.acc[6 * slot + k];
-  const double e_base = m.e_base[r];
+    for (int k = 0; k < 6; ++k) A[k] = gldi(m.acc, 6 * slot + k);
+  const double e_base = gldi(m.e_base, r);
   double w_launch = 0.0;      // thread 0: the launch's accepted dE at the last block sum
   for (int sw = 0; sw < nsweeps; ++sw) {
     const unsigned long long sweep = sweep0 + (unsigned long long)sw;
     for (int c = 0; c < a.ncol; ++c) {
-      const int beg = a.col_off[c], end = a.col_off[c + 1];
+      const int beg = gldi(a.col_off, c), end = gldi(a.col_off, c + 1);
       for (int base = beg; base < end; base += per) {  // uniform trip count in the block
         const int d0 = base + threadIdx.x / tps;
         const bool active = d0 < end;
@@ -446,7 +524,7 @@ __device__ __forceinline__ void lds_body(const KArgs &a, int r, int nsweeps, con
         Proposal p{};
         if (active) p = propose<MODE>(a, sp, st, r, sweep);
         double eJ, eB, eD;
-        bond_sums<kChunk, CMP, HB, HD, SELF, false>(a, sp, ctab, st, nullptr, 0, active, g, p, eJ, eB, eD);
+        bond_sums<kChunkLds, CMP, HB, HD, SELF, false>(a, sp, ctab, st, nullptr, 0, active, g, p, eJ, eB, eD);
         eJ = group_sum(eJ, tps);
         if (HB) eB = group_sum(eB, tps);
         if (HD) eD = group_sum(eD, tps);
@@ -491,22 +569,22 @@ __device__ __forceinline__ void lds_body(const KArgs &a, int r, int nsweeps, con
   }
   if (threadIdx.x == 0) {
     if (nmeas_local) {
-      for (int k = 0; k < 6; ++k) m.acc[6 * slot + k] = A[k];
-      if (r == 0) m.nmeas[0] += nmeas_local;
+      for (int k = 0; k < 6; ++k) gsti(m.acc, 6 * slot + k, A[k]);
+      if (r == 0) gsti(m.nmeas, 0, gldi(m.nmeas, 0) + nmeas_local);
     }
     if (nsweeps > 0) {
-      m.e_run[r] = e_last;
-      m.e_base[r] = e_base + w_launch;   // one block per replica: the only writer
+      gsti(m.e_run, r, e_last);
+      gsti(m.e_base, r, e_base + w_launch);   // one block per replica: the only writer
     }
   }
   double *ox = a.sp + (size_t)r * 3 * a.Ns;
   for (int d = threadIdx.x; d < a.N; d += kLdsBlock) {
     const double4 s = s_sp[d];
-    ox[d] = s.x; ox[a.Ns + d] = s.y; ox[2 * a.Ns + d] = s.z;
+    gsti(ox, d, s.x); gsti(ox, a.Ns + d, s.y); gsti(ox, 2 * (size_t)a.Ns + d, s.z);
   }
   unsigned long long v = nrej;
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(&a.rej[slot], (unsigned long long)v);
+  if ((threadIdx.x & 63) == 0 && v) rej_add(a.rej, slot, v);
 }
 
 template <int MODE, bool CMP, bool HB, bool HD, bool SELF>
@@ -742,6 +820,15 @@ __global__ void k_copy(const double4 *src, double4 *dst, size_t n) {
 __global__ void k_fill(double *p, size_t n, double v) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = v;
+}
+
+// Per-slot totals of the rejection sub-counters rej[slot][kRejSub] (host side).
+inline void sum_rej_sub(const std::vector<int64_t> &sub, int R, int64_t *out) {
+  for (int s = 0; s < R; ++s) {
+    int64_t t = 0;
+    for (int k = 0; k < kRejSub; ++k) t += sub[(size_t)s * kRejSub + k];
+    out[s] = t;
+  }
 }
 
 }  // namespace

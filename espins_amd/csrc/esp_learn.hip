@@ -326,18 +326,18 @@ __device__ __forceinline__ void local_env(const KArgs &a, const SP &sp, const do
   for (int k = 0; k < 6; ++k) QB[k] = 0.0;
   const double *crow = CMP ? ctab + (size_t)a.basis[d] * a.zmax * 5 : nullptr;
   const unsigned ud = (unsigned)d;
-  for (int l0 = 0; l0 < a.zmax; l0 += kChunk) {
-    unsigned jj[kChunk];
-    V3 sv[kChunk];
+  for (int l0 = 0; l0 < a.zmax; l0 += kChunkLds) {
+    unsigned jj[kChunkLds];
+    V3 sv[kChunkLds];
 #pragma unroll
-    for (int u = 0; u < kChunk; ++u) {
+    for (int u = 0; u < kChunkLds; ++u) {
       const int l = l0 + u < a.zmax ? l0 + u : 0;
       jj[u] = (unsigned)ld_off(a.nbr, ((unsigned)l * (unsigned)a.N + ud) * 4u);
     }
 #pragma unroll
-    for (int u = 0; u < kChunk; ++u) sv[u] = sp.ld(jj[u]);
+    for (int u = 0; u < kChunkLds; ++u) sv[u] = sp.ld(jj[u]);
 #pragma unroll
-    for (int u = 0; u < kChunk; ++u) {
+    for (int u = 0; u < kChunkLds; ++u) {
       const int l = l0 + u;
       if (l >= a.zmax) continue;
       const V3 sj = sv[u];
@@ -484,7 +484,7 @@ __global__ __launch_bounds__(kBlock) void k_phase_learned(KArgs a, const uint16_
     sp.x[d] = sn.x; sp.y[d] = sn.y; sp.z[d] = sn.z;
   }
   const unsigned long long m = __ballot(active && !acc);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(&a.rej[slot], (unsigned long long)__popcll(m));
+  if ((threadIdx.x & 63) == 0 && m) rej_add(a.rej, slot, (unsigned long long)__popcll(m));
   Proposal pr;
   pr.so = so; pr.sn = sn; pr.u3 = 0.0;
   wave_sums(a, r, b, active, !acc, dE, pr, 1);   // accepted dE into the wave's accumulator
