@@ -1,18 +1,27 @@
 #!/usr/bin/env python3
-"""Benchmark: attempted spin updates/s of the parallel-tempering Metropolis sweep on one MI355X.
+"""Benchmark: attempted spin updates/s of the Metropolis sweep with parallel tempering on one MI355X.
 
-Workload (default ``--config pt_fcc16``): FCC conventional cell, 16^3 cells (N = 16384 FP64
-3-vector spins), J1 = -1, J2 = +0.2, J3 = +0.05, easy-axis single-ion Delta = -0.1 along z,
-64 replicas on a geometric ladder 0.05-1.5 J, replica exchange after every sweep,
-'random' proposals, uniform-on-sphere initial spins (seed 4242).  One step = one sweep of
-every replica + one measurement of every replica + one exchange round.  Synthetic inputs,
-resident in HBM before the timed region.
+Default workload (``--config batch_c3``): 256 independent PT chains advanced together --
+lattice SC / BCC (2 basis) / FCC (4 basis) and L in {8, 12, 16, 24, 32, 48} drawn per chain,
+16-32 replicas on a geometric 0.5-2.5 J ladder, J1 = 1, J2 ~ U[-0.3, 0.3], 'random' proposals,
+uniform-on-sphere initial spins, all from master seed 7 (472,136,384 spin updates per sweep of
+the batch).  One step = one sweep of every replica of every chain + one measurement of every
+replica + one exchange round of every chain.  Synthetic inputs, resident in HBM before the
+timed region.
+
+Other workloads: pt_fcc16 (FCC 16^3, J1-J3 + Delta, 64 replicas), mc_sc8, pt_sc64_dm (SC 64^3,
+J + B + DM + Delta, 32 replicas), pt_fcc32 (FCC 32^3, J1-J2 + B + Delta, 64 replicas),
+learned_fcc32 (learned proposal vs plain Metropolis, effective samples/s of E), copy_probe.
 
 Prints one JSON line (stdout).  Extra legs:
-  roofline      -- sweep-kernel bytes per launch (algorithmic, gather model) / mean launch
-                   time, timed with HIP events on the engine's stream
+  roofline      -- the dominant sweep kernel: minimal algorithmic bytes of its launches (each
+                   spin a colour phase reads, read once; each updated spin, written once) / the
+                   launches' summed duration, timed with HIP events on the engine's stream;
+                   traffic = HBM bytes per launch from the committed rocprofv3 counter summary
   cpu_baseline  -- the oracle's reference-order sweep (MT19937, sequential sites, FP64),
                    one process per host core, on a bounded sample of the same workload
+  extras        -- (batch_c3) short runs of pt_fcc16 / pt_sc64_dm / pt_fcc32, the copy
+                   bandwidth, and the learned-proposal ESS ratios of the committed profile
 """
 from __future__ import annotations
 
@@ -306,8 +315,53 @@ def run_batch(args):
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
-    print(json.dumps(out), flush=True)
     b.close()
+    if not args.no_extras:
+        extras_into(out, args)
+    print(json.dumps(out), flush=True)
+
+
+def profile_ess():
+    """Learned-proposal ESS ratios from the committed profile of the learned_fcc32 run."""
+    import ast
+    path = os.path.join(ROOT, "profiles", "ess_learned_vs_plain_fcc32.txt")
+    if not os.path.exists(path):
+        return None
+    got = {}
+    for line in open(path):
+        t = line.strip()
+        for key in ("ess_ratio_vs_random", "ess_ratio_vs_const"):
+            if t.startswith(key + " ="):
+                got[key] = ast.literal_eval(t.split("=", 1)[1].strip())
+    if not got:
+        return None
+    got["source"] = "profiles/ess_learned_vs_plain_fcc32.txt (python bench.py --config learned_fcc32)"
+    return got
+
+
+def extras_into(out, args):
+    """Short single-ladder runs of the other workloads and the copy bandwidth, as extra keys
+    of the batched line (no CPU legs; the same sweep path and timing as --config <name>)."""
+    from espins_amd.engine import Engine
+    ex = {}
+    for cfg, steps, warm in (("pt_fcc16", 2000, 200), ("pt_sc64_dm", 200, 20), ("pt_fcc32", 300, 30)):
+        r = run_single(args, cfg, steps, warm, with_cpu=False, emit=False)
+        rf = r["roofline"]
+        ex[cfg] = {"workload": r["config"]["workload"], "value": r["value"], "unit": "updates/s",
+                   "ms_per_step": r["ms_per_step"], "steps": steps, "kernel": rf["kernel"],
+                   "mean_launch_us": rf["mean_launch_us"], "roofline_frac": rf["frac"],
+                   "bytes_per_update": rf["bytes_per_update"]}
+    out["extras"] = ex
+    ess = profile_ess()
+    if ess:
+        out["learned_ess_ratio"] = ess
+    eng = Engine(0)
+    copy_gbs = 2 * (1 << 28) / (eng.copy_probe(1 << 28, reps=10) * 1e-3) / 1e9
+    eng.close()
+    if out.get("roofline"):
+        out["roofline"]["copy_bandwidth_GBs"] = copy_gbs
+        out["roofline"]["frac_of_copy"] = out["roofline"]["achieved"] / copy_gbs
+    log(f"copy bandwidth {copy_gbs:.0f} GB/s")
 
 
 # ---------------------------------------------------------------- learned proposal --------
@@ -384,7 +438,7 @@ def run_learned(args):
     ratio = {k: res["learned"]["ess_s"] / res[k]["ess_s"] for k in ("random", "const")}
     reliable = int(np.sum(res["random"]["tau"] * 50 < args.steps))
     tm = eng.time_sweeps(20, mode="learned", seed=seed + 1, sweep0=10 ** 9)
-    B = bytes_per_update(m)
+    B = phase_bytes(m)[0] / m.N                  # minimal bytes per update, mean over colours
     ms_launch = tm["kernel_ms"] / tm["nlaunch"]
     achieved = tm["updates_per_launch"] * B / (ms_launch * 1e-3) / 1e9
     log(f"k_phase_learned: mean {ms_launch * 1e3:.1f} us per launch, "
@@ -411,7 +465,8 @@ def run_learned(args):
         "train": {"loss_first": float(losses[0]), "loss_last": float(losses[-1]), "seconds": t_train},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_phase_learned",
-                     "bytes_per_update": B, "updates_per_launch": tm["updates_per_launch"],
+                     "bytes_per_update": B, "model": "minimal colour-phase traffic",
+                     "updates_per_launch": tm["updates_per_launch"],
                      "mean_launch_us": ms_launch * 1e3},
         "cpu_baseline": cpu,
     }
@@ -448,49 +503,19 @@ def measured_traffic(config, kernel):
 
 
 # ---------------------------------------------------------------- main ---------------------
-def main():
-    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000)
-    ap.add_argument("--warmup", type=int, default=1000)
-    ap.add_argument("--config", default="pt_fcc16")
-    ap.add_argument("--path", type=int, default=0, help="0 auto, 1 global-memory phases, 2 LDS")
-    ap.add_argument("--tps", type=int, default=0, help="lanes per site (0 automatic)")
-    ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--chains", type=int, default=256, help="batch_c3: number of chains")
-    ap.add_argument("--tune-cache", default=None, help="batch_c3: autotuner cache file")
-    ap.add_argument("--batch-path", default="auto", choices=("auto", "global", "fit"),
-                    help="batch_c3: autotuned kernel shapes, every chain on the global-memory "
-                         "path, or LDS wherever a replica fits")
-    ap.add_argument("--records", type=int, default=1 << 20, help="learned_fcc32: training records")
-    ap.add_argument("--epochs", type=int, default=4, help="learned_fcc32: training epochs")
-    ap.add_argument("--tilt", type=float, default=0.125, help="learned_fcc32: 'const' cone tilt")
-    ap.add_argument("--ess-warmup", type=int, default=500, help="learned_fcc32: sweeps before the trace")
-    ap.add_argument("--start", default="ordered", choices=("ordered", "random"),
-                    help="learned_fcc32: initial spins")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; --gpus must be 1 "
-              "(multi-GPU sharding of replica ladders is described in DESIGN.md, not built)",
-              file=sys.stderr)
-        sys.exit(2)
+STEP_DEFAULTS = {"batch_c3": (200, 20), "learned_fcc32": (2000, 2000), "copy_probe": (3, 0)}
 
-    if args.config == "batch_c3":
-        return run_batch(args)
-    if args.config == "learned_fcc32":
-        return run_learned(args)
-    if args.config == "copy_probe":
-        return run_copy_probe(args)
-    m, tems, desc, seed = workload(args.config)
+
+def run_single(args, config, steps, warmup, with_cpu=True, emit=True):
+    """One single-context workload (plain MC or PT of one replica ladder); returns the JSON
+    line as a dict (and prints it when emit)."""
+    m, tems, desc, seed = workload(config)
     cpu = None
-    if not args.no_cpu:   # before any GPU call (forked workers must not inherit a GPU context)
+    if with_cpu:   # before any GPU call (forked workers must not inherit a GPU context)
         log(f"cpu_baseline: {args.cpu_seconds:.0f} s per core ...")
-        cpu = cpu_baseline(args.config, tems, args.cpu_seconds)
+        cpu = cpu_baseline(config, tems, args.cpu_seconds)
         log(f"cpu_baseline: {cpu['value'] / 1e6:.2f} M updates/s on {cpu['cores']} cores")
 
-    import numpy as np
     from espins_amd import lattice as Lt
     from espins_amd.engine import Engine
     # No torch here: its wheel carries its own HIP runtime, and a second runtime in the
@@ -504,8 +529,8 @@ def main():
     eng.set_tps(args.tps)
     eng.set_replicas(tems, Lt.random_spins(m.N, R, seed, m.mag))
     info = eng.model_info()
-    log(f"model: {info}")
-    pt = R > 1 and args.config.startswith("pt")
+    log(f"{config} model: {info}")
+    pt = R > 1 and config.startswith("pt")
     graph = not args.no_graph
 
     def run(n, sweep0, step0):
@@ -517,48 +542,46 @@ def main():
             eng.run(1, steps_per_round=n, steps_measure=1, measure=True, exchange=False,
                     mode="random", seed=seed, sweep0=sweep0, graph=False)
 
-    run(args.warmup, 0, 0)
+    run(warmup, 0, 0)
     eng.reset_stats()
     eng.sync()
     t0 = time.perf_counter()
-    run(args.steps, args.warmup, args.warmup)
+    run(steps, warmup, warmup)
     eng.sync()
     dt = time.perf_counter() - t0
-    updates = m.N * R * args.steps
+    updates = m.N * R * steps
     value = updates / dt
     st = eng.stats()
     acc_rate = 1.0 - st["rejected"].sum() / float(updates)
     sr, ns = eng.pt_map()
     swap_rate = float(ns[:, 1].sum()) / max(1, ns[:, 0].sum())
-    log(f"timed: {args.steps} steps in {dt:.3f} s -> {value / 1e9:.3f} G updates/s; "
+    log(f"{config} timed: {steps} steps in {dt:.3f} s -> {value / 1e9:.3f} G updates/s; "
         f"acceptance {acc_rate:.4f}, swap acceptance {swap_rate:.4f}")
 
     # sweep-kernel roofline (HIP events around each launch on the engine's stream)
-    nsw = max(10, min(200, args.steps // 10))
+    nsw = max(10, min(200, steps // 10))
     tm = eng.time_sweeps(nsw, mode="random", seed=seed + 1, sweep0=10 ** 9)
-    B = bytes_per_update(m)
+    pb = phase_bytes(m)[0] * R                  # minimal bytes of one sweep of every replica
     ms_launch = tm["kernel_ms"] / tm["nlaunch"]
-    alg_bytes = tm["updates_per_launch"] * B
-    achieved = alg_bytes / (ms_launch * 1e-3) / 1e9
+    launches_per_sweep = tm["nlaunch"] / nsw
+    alg_launch = pb / launches_per_sweep
+    achieved = alg_launch / (ms_launch * 1e-3) / 1e9
     kernel = "k_lds" if info["lds"] else "k_phase"
+    G = bytes_per_update(m)
     log(f"{kernel}: {tm['nlaunch']} launches, mean {ms_launch * 1e3:.2f} us, "
-        f"{tm['updates_per_launch']} updates/launch, B_upd {B:.0f} B -> {achieved:.0f} GB/s")
-    # measured copy bandwidth of this device (read + write of 256 MiB), and the FP64 work of
-    # the sweep (70 + 6 zJ + 12 zB + 11 zD flops per update) against the FP64 vector peak
-    copy_gbs = 2 * (1 << 28) / (eng.copy_probe(1 << 28, reps=10) * 1e-3) / 1e9
+        f"{tm['updates_per_launch']} updates/launch, {pb / (m.N * R):.0f} B/update minimal "
+        f"-> {achieved:.0f} GB/s")
     zJ, zB, zD = m.J.z, (m.B.z if m.B else 0), (m.D.z if m.D else 0)
     flops = 70 + 6 * zJ + 12 * zB + 11 * zD
-    log(f"copy bandwidth {copy_gbs:.0f} GB/s; sweep FP64 work {flops} flop/update -> "
-        f"{value * flops / 1e12:.2f} TFLOP/s of 78.6")
-
+    traffic = measured_traffic(config, kernel)
     out = {
         "metric": METRIC,
         "value": value,
         "unit": "updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": dt / args.steps * 1e3,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": dt / steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -568,18 +591,72 @@ def main():
                    "exchange_every": 1 if pt else 0, "mode": "random", "colours": info["ncol"],
                    "kernel_path": "lds" if info["lds"] else "global", "graph": graph},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.config, kernel),
-                     "kernel": kernel, "bytes_per_update": B,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": (f"profiles/pmc_{kernel}_{config}.txt (FETCH_SIZE + WRITE_SIZE, "
+                                        "copy-kernel calibrated)") if traffic else None,
+                     "kernel": kernel, "bytes_per_update": pb / (m.N * R),
+                     "model": "minimal colour-phase traffic (each spin a phase reads once, each "
+                              "updated spin written once, 24 B per FP64 3-vector)",
                      "updates_per_launch": tm["updates_per_launch"],
-                     "mean_launch_us": ms_launch * 1e3, "copy_bandwidth_GBs": copy_gbs,
-                     "frac_of_copy": achieved / copy_gbs,
+                     "mean_launch_us": ms_launch * 1e3,
+                     "gather_model": {"bytes_per_update": G,
+                                      "GBs": tm["updates_per_launch"] * G / (ms_launch * 1e-3) / 1e9},
                      "fp64_flops_per_update": flops, "fp64_frac": value * flops / 78.6e12},
         "cpu_baseline": cpu,
         "acceptance": acc_rate,
         "swap_acceptance": swap_rate,
     }
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
     eng.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default per config)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default per config)")
+    ap.add_argument("--config", default="batch_c3",
+                    choices=("batch_c3", "pt_fcc16", "mc_sc8", "pt_sc64_dm", "pt_fcc32",
+                             "learned_fcc32", "copy_probe"))
+    ap.add_argument("--path", type=int, default=0, help="0 auto, 1 global-memory phases, 2 LDS")
+    ap.add_argument("--tps", type=int, default=0, help="lanes per site (0 automatic)")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="batch_c3: skip the short single-ladder runs reported as extra keys")
+    ap.add_argument("--chains", type=int, default=256, help="batch_c3: number of chains")
+    ap.add_argument("--tune-cache", default=None, help="batch_c3: autotuner cache file")
+    ap.add_argument("--batch-path", default="global", choices=("auto", "global", "fit"),
+                    help="batch_c3: every chain on the global-memory path (default), autotuned "
+                         "kernel shapes, or LDS wherever a replica fits")
+    ap.add_argument("--records", type=int, default=1 << 20, help="learned_fcc32: training records")
+    ap.add_argument("--epochs", type=int, default=4, help="learned_fcc32: training epochs")
+    ap.add_argument("--tilt", type=float, default=0.125, help="learned_fcc32: 'const' cone tilt")
+    ap.add_argument("--ess-warmup", type=int, default=500, help="learned_fcc32: sweeps before the trace")
+    ap.add_argument("--start", default="ordered", choices=("ordered", "random"),
+                    help="learned_fcc32: initial spins")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus must be 1 "
+              "(sharding replica ladders over GPUs is not built)",
+              file=sys.stderr)
+        sys.exit(2)
+    st, wu = STEP_DEFAULTS.get(args.config, (10000, 1000))
+    if args.steps is None:
+        args.steps = st
+    if args.warmup is None:
+        args.warmup = wu
+
+    if args.config == "batch_c3":
+        return run_batch(args)
+    if args.config == "learned_fcc32":
+        return run_learned(args)
+    if args.config == "copy_probe":
+        return run_copy_probe(args)
+    run_single(args, args.config, args.steps, args.warmup, with_cpu=not args.no_cpu)
 
 
 if __name__ == "__main__":
