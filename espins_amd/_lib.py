@@ -80,7 +80,16 @@ _SIGS = {
     "esp_sync": (_i, [_p]),
     "esp_stream": (_p, [_p]),
     "esp_copy_probe": (_i, [_p, C.c_int64, _i, _dp]),
+    "esp_set_orderparam": (_i, [_p, _dp]),
+    "esp_set_staggered": (_i, [_p, _i, _dp]),
+    "esp_set_pm": (_i, [_p, _i, _ip]),
+    "esp_set_spincorr": (_i, [_p, _i, _i, _ip, _i, _ip, _ip]),
+    "esp_set_sfactor": (_i, [_p, _i, _dp, _dp, _dp, _i]),
+    "esp_get_obs": (_i, [_p, _i, _dp, _i64p]),
+    "esp_get_obs_trace": (_i, [_p, _dp, _ip]),
 }
+
+OBS_ORDER, OBS_STAGGERED, OBS_PM, OBS_SPINCORR, OBS_SFACTOR = 1, 2, 4, 8, 16
 
 _fp = C.POINTER(C.c_float)
 _SIGS.update({

@@ -72,6 +72,36 @@ struct esp_ctx {
   int pgrad_cap = 0, loss_cap = 0;
   int *d_perm = nullptr;
   int perm_cap = 0;
+  // optional observables (esp_obs.hip), measured with the energy and m at every measurement
+  // of the global-memory path; accumulators by slot, cleared with the statistics
+  struct Obs {
+    int flags = 0;                 // ESP_OBS_* bits that are on
+    // order parameter / staggered magnetisation: [nb][3] axes, [nb] coefficients
+    double *d_axes = nullptr, *d_coeff = nullptr;
+    double *d_acc = nullptr;       // [R][6]: |OP|, OP^2, OP^4, |SM|, SM^2, SM^4
+    double *d_part = nullptr;      // [R][cell blocks][4]
+    int cblocks = 0;
+    // P(m): grid m_min + k (m_max - m_min) / (nfreq - 1); counts [R][3][nfreq]
+    int nfreq = 0;
+    double m_min = 0.0, m_max = 0.0;
+    unsigned long long *d_pm = nullptr;
+    // spin correlations per shell: tables by reference site
+    int nshell = 0, nsp = 0, zs = 0;
+    int *d_species = nullptr, *d_scnt = nullptr, *d_snbr = nullptr;
+    double *d_snorm = nullptr;     // [nshell][nsp][nsp]
+    double *d_sc = nullptr;        // [R][3][nshell][nsp][nsp]: sum, |sum|, sum |.|
+    double *d_scpart = nullptr;    // [R][site blocks][nshell][2][nsp][nsp]
+    int sblocks = 0;
+    // structure factor
+    int nq = 0, sf_every = 1;
+    double polar[3] = {0, 0, 0};
+    int have_polar = 0;
+    double *d_q = nullptr, *d_pos = nullptr;   // [nq][3], [N][3] by reference site
+    double *d_sf = nullptr;        // [R][3][nq]
+    unsigned long long *d_sfn = nullptr;      // structure-factor measurements
+    // per-measurement |OP| and |SM| by slot while the trace is on: [trace_cap][R][2]
+    double *d_trace = nullptr;
+  } obs;
   // graph cache
   hipGraphExec_t gexec = nullptr;
   std::string gkey;
@@ -169,6 +199,13 @@ KArgs base_args(esp_ctx *c) {
 }  // namespace
 
 namespace esp {
+// Optional observables (esp_obs.hip): the measurement kernels of one measurement, enqueued
+// after the partial sums of the energy and m and before the finalisation (which counts the
+// measurement); state freed with the replicas / the model; accumulators cleared.
+int obs_enqueue(esp_ctx *c);
+void obs_free(esp_ctx *c);
+int obs_resize(esp_ctx *c);      // (re)allocate the per-replica accumulators for c->R
+int obs_clear(esp_ctx *c);
 // Learned-proposal state (esp_learn.hip).
 // ev (optional): 2 events per launch, recorded by the dispatch packets (timing); *nlaunch
 // receives the number of launches.

@@ -294,7 +294,7 @@ int rebase(esp_ctx *c) {
 // The LDS-resident kernel serves the plain proposals when the replica fits and no trace
 // is recorded; learned proposals and traced runs use the global-memory path.
 bool lds_sweeps(const esp_ctx *c, int mode) {
-  return c->use_lds && mode != ESP_MODE_LEARNED && !c->d_trace;
+  return c->use_lds && mode != ESP_MODE_LEARNED && !c->d_trace && !c->obs.flags;
 }
 
 // One MC "round": steps x (M sweeps [+ measure]) [+ exchange]; advances the clock.
@@ -327,6 +327,7 @@ int enqueue_round(esp_ctx *c, int steps, int M, bool measure, bool exchange, int
     if (rc) return rc;
     rc = enqueue_reduce(c);
     if (rc) return rc;
+    if ((rc = esp::obs_enqueue(c))) return rc;
     const bool last = s + 1 == steps;
     rc = enqueue_finalize(c, true, true, exchange && last, last ? steps * M : 0, seed);
     if (rc) return rc;
@@ -378,6 +379,7 @@ int esp_destroy(esp_ctx *c) {
   (void)hipStreamSynchronize(c->stream);
   free_replicas(c);
   free_model(c);
+  esp::obs_free(c);
   esp::learn_free(c);
   dfree(c->d_clock);
   (void)hipStreamDestroy(c->stream);
@@ -434,6 +436,7 @@ int esp_set_model(esp_ctx *c, int N, int nb, const double *mag, int zJ, const in
     return fail(c, ESP_EINVAL, "bond table or replica beyond 4 GiB (32-bit device offsets)");
   free_replicas(c);
   free_model(c);
+  esp::obs_free(c);             // observables belong to a model: all off
   c->hm = std::move(hm);
   const esp::HostModel &m = c->hm;
   int rc;
@@ -608,6 +611,7 @@ int esp_set_replicas(esp_ctx *c, int R, const double *tems, const double *spins)
   HIPCHK(c, hipMemsetAsync(c->d_rej, 0, sizeof(unsigned long long) * kRejSub * R, c->stream));
   HIPCHK(c, hipMemsetAsync(c->d_nswap, 0, sizeof(unsigned long long) * 2 * R, c->stream));
   HIPCHK(c, hipMemsetAsync(c->d_nmeas, 0, sizeof(unsigned long long), c->stream));
+  if ((rc = esp::obs_resize(c))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return esp_put_spins(c, spins);
 }
@@ -907,7 +911,7 @@ int esp_set_trace(esp_ctx *c, int capacity) {
     c->trace_cap = capacity;
   }
   drop_graph(c);
-  return ESP_OK;
+  return esp::obs_resize(c);
 }
 
 int esp_get_trace(esp_ctx *c, double *trace, int *n) {
@@ -935,6 +939,7 @@ int esp_reset_stats(esp_ctx *c) {
   HIPCHK(c, hipMemsetAsync(c->d_rej, 0, sizeof(unsigned long long) * kRejSub * R, c->stream));
   HIPCHK(c, hipMemsetAsync(c->d_nswap, 0, sizeof(unsigned long long) * 2 * R, c->stream));
   HIPCHK(c, hipMemsetAsync(c->d_nmeas, 0, sizeof(unsigned long long), c->stream));
+  if ((rc = esp::obs_clear(c))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return ESP_OK;
 }

@@ -186,6 +186,59 @@ class Engine:
                                 float(tilt), int(seed) & (2**64 - 1), int(sweep0), int(step0),
                                 int(graph)))
 
+    # ---- optional observables (measured at every measurement, by slot) ---------------
+    def set_orderparam(self, axes):
+        """axes[nb][3] (orderparam_axes_cart) or None to switch off."""
+        a = None if axes is None else E.f64(axes)
+        self._chk(E.lib.esp_set_orderparam(self._h, E.ptr(a, C.c_double)))
+
+    def set_staggered(self, coeff):
+        """coeff[nb] (staggered_m_coeff) or None to switch off."""
+        c = None if coeff is None else E.f64(coeff)
+        self._chk(E.lib.esp_set_staggered(self._h, int(c is not None), E.ptr(c, C.c_double)))
+
+    def set_pm(self, enable=True):
+        """P(m) histograms of the cell moments; returns the grid size nfreq."""
+        n = C.c_int(0)
+        self._chk(E.lib.esp_set_pm(self._h, int(enable), C.byref(n)))
+        return n.value
+
+    def set_spincorr(self, species, cnt, nbr):
+        """species[N] (0-based kinds), cnt[nshell][N], nbr[nshell][zs][N] (1-based ids), or
+        species=None to switch off."""
+        if species is None:
+            self._chk(E.lib.esp_set_spincorr(self._h, 0, 0, None, 0, None, None))
+            return
+        sp, ct, nb = E.i32(species), E.i32(cnt), E.i32(nbr)
+        nshell, zs = nb.shape[0], nb.shape[1]
+        self._chk(E.lib.esp_set_spincorr(self._h, nshell, int(sp.max()) + 1, E.ptr(sp, C.c_int), zs,
+                                         E.ptr(ct, C.c_int), E.ptr(nb, C.c_int)))
+
+    def set_sfactor(self, q, pos, polar=None, every=1):
+        """q[nq][3] and pos[N][3] (Cartesian), polar[3] or None; q=None switches off."""
+        if q is None:
+            self._chk(E.lib.esp_set_sfactor(self._h, 0, None, None, None, 1))
+            return
+        qv, pv = E.f64(q), E.f64(pos)
+        pl = None if polar is None else E.f64(polar)
+        self._chk(E.lib.esp_set_sfactor(self._h, len(qv), E.ptr(qv, C.c_double), E.ptr(pv, C.c_double),
+                                        E.ptr(pl, C.c_double), int(every)))
+
+    def obs(self, which, shape):
+        """Accumulator of one observable (see esp_get_obs) as an array of `shape` (leading
+        dimension R, by slot) and its measurement count."""
+        out = np.zeros(shape)
+        n = C.c_int64(0)
+        self._chk(E.lib.esp_get_obs(self._h, int(which), E.ptr(out, C.c_double), C.byref(n)))
+        return out, n.value
+
+    def obs_trace(self):
+        n = C.c_int(0)
+        self._chk(E.lib.esp_get_obs_trace(self._h, None, C.byref(n)))
+        out = np.zeros((n.value, self.R, 2))
+        self._chk(E.lib.esp_get_obs_trace(self._h, E.ptr(out, C.c_double), C.byref(n)))
+        return out
+
     def time_sweeps(self, nsweeps, mode="random", tilt=0.125, seed=1, sweep0=0):
         """HIP-event timing of the sweep kernels (see esp_time_sweeps)."""
         kms = C.c_double()

@@ -186,6 +186,39 @@ def cubic_model(kind: str, L: int, J=(1.0,), B: float = 0.0, D: float = 0.0,
     return m
 
 
+def shell_tables(kind: str, L: int, nshell: int, builder=None):
+    """Neighbour tables of the first nshell distance shells of a cubic lattice (the shells of
+    the J rows that spin correlations are reported for, jij.F90:449-503): cnt[nshell][N] and
+    nbr[nshell][zs][N] (1-based site ids, zs the widest shell)."""
+    nb = len(BASIS[kind])
+    bf = np.array(BASIS[kind], float)
+    Ls = (L, L, L)
+    tabs = []
+    for k in range(nshell):
+        c = [0.0] * nshell
+        c[k] = 1.0
+        tabs.append(build_table(nb, bf, np.ones(nb), Ls, shell_rows(kind, c), 1, builder=builder))
+    N = L ** 3 * nb
+    zs = max(t.z for t in tabs)
+    cnt = np.zeros((nshell, N), np.int32)
+    nbr = np.zeros((nshell, zs, N), np.int32)
+    for k, t in enumerate(tabs):
+        cnt[k] = t.cnt
+        nbr[k, :t.z] = t.nbr
+    return cnt, nbr
+
+
+def site_positions(kind: str, L: int) -> np.ndarray:
+    """Cartesian positions [N][3] (cubic cell edge 1) in the reference site order
+    ((x2 L + x1) L + x0) nb + a."""
+    bf = np.array(BASIS[kind], float)
+    nb = len(bf)
+    x = np.arange(L)
+    x2, x1, x0 = np.meshgrid(x, x, x, indexing="ij")
+    cells = np.stack([x0.ravel(), x1.ravel(), x2.ravel()], axis=1).astype(float)
+    return (cells[:, None, :] + bf[None, :, :]).reshape(-1, 3)
+
+
 def ladder(mode: str, t0: float, t1: float, n: int) -> np.ndarray:
     """Temperature ladders of parameters.F90:293-317 ('lin', 'inv', 'log')."""
     t = np.empty(n)

@@ -178,6 +178,50 @@ int esp_reset_stats(esp_ctx *ctx);
  * sweeps use the global-memory path.  trace[n][R][2] by slot; *n = measurements stored. */
 int esp_set_trace(esp_ctx *ctx, int capacity);
 int esp_get_trace(esp_ctx *ctx, double *trace, int *n);
+/* ---- optional observables (measured on the device at every measurement) ------------
+ * Each setter switches one observable on (or off with a null table / zero size) for the
+ * current model and replicas (set them again after esp_set_model / esp_set_replicas).
+ * They are measured with E and m at every measurement of esp_run (the measured steps of
+ * mcarlo.F90:526-590 and pt.F90:715-908), accumulated by slot (temperature), cleared by
+ * esp_reset_stats.  While any is on, sweeps use the global-memory path.
+ * Sites are in the reference order; a cell is nb consecutive sites. */
+#define ESP_OBS_ORDER     1   /* order parameter, orderparam_get (get_quant.F90:396-422)  */
+#define ESP_OBS_STAGGERED 2   /* staggered magnetisation, staggered_get (get_quant.F90:366-392) */
+#define ESP_OBS_PM        4   /* P(m), Pm_get (get_quant.F90:425-493)                      */
+#define ESP_OBS_SPINCORR  8   /* spin correlations per shell, spincorr_get (get_quant.F90:496-540) */
+#define ESP_OBS_SFACTOR  16   /* structure factor, mcarlo_structure_factor (get_quant.F90:649-717) */
+/* axes[nb][3] (orderparam_axes_cart, parameters.F90); OP = sum_i S_i . axes[basis i] / N. */
+int esp_set_orderparam(esp_ctx *ctx, const double *axes);
+/* coeff[nb] (staggered_m_coeff); SM = | sum_i coeff[basis i] S_i / N |. */
+int esp_set_staggered(esp_ctx *ctx, int enable, const double *coeff);
+/* P(m) of the cell moments |sum of a cell's spins| (and of the cell order parameter and
+ * cell staggered moment when those are on) on the grid of mcarlo.F90:432-456:
+ * m_max = sum_a |moment_a|, m_min = -m_max, nfreq = nint((m_max - m_min) / 0.01) + 1
+ * (at least 2).  *nfreq receives the grid size. */
+int esp_set_pm(esp_ctx *ctx, int enable, int *nfreq);
+/* Spin correlations per shell (shell_neighbors_get, jij.F90:449-503): species[N] (0-based
+ * kind of every site), cnt[nshell][N] and nbr[nshell][zs][N] (1-based site ids): the
+ * neighbours of each site in each shell.  The normalisation is the reference's
+ * shell_num_nbors_kind (its per-kind count of the last site of each kind). */
+int esp_set_spincorr(esp_ctx *ctx, int nshell, int nspecies, const int *species, int zs,
+                     const int *cnt, const int *nbr);
+/* Structure factor on nq q-points q[nq][3] (Cartesian, the phase is q . r) for site
+ * positions pos[N][3] (Cartesian), measured when the measurement count (from 1) is a
+ * multiple of every (sfactor_steps_measure); polar[3] (sfactor_polar) or NULL. */
+int esp_set_sfactor(esp_ctx *ctx, int nq, const double *q, const double *pos, const double *polar,
+                    int every);
+/* Accumulators by slot (sums over the measurements; divide by the counts):
+ *   ESP_OBS_ORDER, ESP_OBS_STAGGERED: out[R][3] = sum |X|, sum X^2, sum X^4
+ *   ESP_OBS_PM:       out[R][ndim][nfreq] counts (as doubles) of the cell values per bin,
+ *                     ndim = 1 + (order on) + (staggered on): m, then OP, then SM
+ *   ESP_OBS_SPINCORR: out[R][3][nshell][nsp][nsp] = sum of spincorr_tmp, of |spincorr_tmp|,
+ *                     of spincorr_abs_tmp (get_quant.F90:517-536)
+ *   ESP_OBS_SFACTOR:  out[R][3][nq] = sum |F|^2, |F|^2 - NSF^2, NSF^2; *count = measurements
+ * count receives the number of measurements that contributed (may be NULL). */
+int esp_get_obs(esp_ctx *ctx, int which, double *out, int64_t *count);
+/* Per-measurement |OP| and |SM| by slot while the trace is on: trace[n][R][2]. */
+int esp_get_obs_trace(esp_ctx *ctx, double *trace, int *n);
+
 /* Wait for the context's stream. */
 int esp_sync(esp_ctx *ctx);
 /* The context's HIP stream (hipStream_t) for callers that time or order work on it. */

@@ -1,0 +1,131 @@
+"""The optional observables of the HIP engine (order parameter, staggered moment, P(m), spin
+correlations per shell, structure factor, their per-measurement trace) against the oracle
+(oracle/obs_oracle.py, pinned by closed forms in test_obs_oracle.py) on the configurations
+the engine measured: one measured MC step per run call, the spins read back after it, the
+oracle's values accumulated by slot exactly as the reference accumulates them per measurement.
+Bars: bin counts exact; sums 1e-12 relative (order of the floating-point sums differs)."""
+import math
+
+import numpy as np
+import pytest
+
+from espins_amd import lattice as Lt
+from oracle import obs_oracle as B
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _setup(exchange_every=False):
+    from espins_amd.engine import Engine
+    L = 4
+    m = Lt.cubic_model("bcc", L, J=(1.0, 0.3))
+    N, nb = m.N, m.nb
+    tems = Lt.ladder("log", 0.4, 3.0, 6)
+    eng = Engine(0)
+    eng.set_model(m)
+    eng.set_replicas(tems, Lt.random_spins(N, len(tems), 5, m.mag))
+    axes = np.array([(0.0, 0.0, 1.0), (0.6, 0.0, -0.8)])
+    coeff = np.array([1.0, -1.0])
+    eng.set_orderparam(axes)
+    eng.set_staggered(coeff)
+    nfreq = eng.set_pm(True)
+    cnt, nbr = Lt.shell_tables("bcc", L, 2)
+    kinds = np.arange(N) % 2
+    eng.set_spincorr(kinds, cnt, nbr)
+    pos = Lt.site_positions("bcc", L)
+    tp = 2 * math.pi
+    rng = np.random.default_rng(3)
+    q = np.concatenate([[(0.0, 0.0, 0.0), (tp, 0, 0), (tp / 2, tp / 2, tp / 2)],
+                        rng.uniform(-2 * tp, 2 * tp, size=(37, 3))])
+    polar = np.array([0.3, -0.2, 0.9])
+    eng.set_sfactor(q, pos, polar, every=2)
+    return eng, m, dict(axes=axes, coeff=coeff, nfreq=nfreq, cnt=cnt, nbr=nbr, kinds=kinds, pos=pos,
+                        q=q, polar=polar)
+
+
+def _oracle_measure(s, m, c):
+    op = B.orderparam(s, m.nb, c["axes"])
+    sm = B.staggered(s, m.nb, c["coeff"])
+    h = B.pm_counts(s, m.nb, m.mag[:m.nb], axes=c["axes"], coeff=c["coeff"])
+    t, ta = B.spincorr(s, c["kinds"], c["cnt"], c["nbr"], m.mag, 2)
+    return op, sm, h, t, ta
+
+
+@pytest.mark.parametrize("exchange", [False, True])
+def test_observables_match_oracle(exchange):
+    eng, m, c = _setup()
+    R = eng.R
+    steps = 6
+    eng.set_trace(steps)
+    acc = np.zeros((R, 6))
+    pm = np.zeros((R, 3, c["nfreq"]))
+    sc = np.zeros((R, 3, 2, 2, 2))
+    sf = np.zeros((R, 3, len(c["q"])))
+    nsf = 0
+    tr = np.zeros((steps, R, 2))
+    for k in range(steps):
+        say(f"step {k} exchange={exchange}")
+        slot_of = np.argsort(eng.pt_map()[0])    # replica -> slot before the round
+        eng.run(1, steps_per_round=1, steps_measure=1, measure=True, exchange=exchange,
+                start_parity=1 + k % 2, seed=11, sweep0=k, step0=k)
+        s_all = eng.spins(by_slot=False)         # by replica (exchanges move labels only)
+        for r in range(R):
+            sl = slot_of[r]
+            op, sm, h, t, ta = _oracle_measure(s_all[r], m, c)
+            acc[sl] += [abs(op), op * op, op ** 4, sm, sm * sm, sm ** 4]
+            pm[sl] += h
+            sc[sl, 0] += t
+            sc[sl, 1] += np.abs(t)
+            sc[sl, 2] += ta
+            tr[k, sl] = [abs(op), sm]
+            if (k + 1) % 2 == 0:
+                sf[sl] += B.sfactor(s_all[r], c["pos"], c["q"], c["polar"])
+        if (k + 1) % 2 == 0:
+            nsf += 1
+    got_op, n_op = eng.obs(1, (R, 3))
+    got_sm, _ = eng.obs(2, (R, 3))
+    got_pm, _ = eng.obs(4, (R, 3, c["nfreq"]))
+    got_sc, _ = eng.obs(8, (R, 3, 2, 2, 2))
+    got_sf, n_sf = eng.obs(16, (R, 3, len(c["q"])))
+    got_tr = eng.obs_trace()
+    say("op", got_op[:, 0], acc[:, 0])
+    assert n_op == steps and n_sf == nsf
+    assert np.allclose(got_op, acc[:, 0:3], rtol=1e-12, atol=0)
+    assert np.allclose(got_sm, acc[:, 3:6], rtol=1e-12, atol=0)
+    assert np.array_equal(got_pm, pm)
+    assert np.allclose(got_sc, sc, rtol=1e-12, atol=1e-12 * m.N)
+    scale = float(m.N) ** 2
+    assert np.allclose(got_sf, sf, rtol=1e-10, atol=1e-12 * scale)
+    assert got_tr.shape == (steps, R, 2)
+    assert np.allclose(got_tr, tr, rtol=1e-12, atol=0)
+    # the running energy is unaffected by the measurement kernels
+    st = eng.stats()
+    assert np.allclose(st["e_run"], eng.energy_get(), rtol=1e-10, atol=1e-10)
+    eng.reset_stats()
+    z, n = eng.obs(1, (R, 3))
+    assert n == 0 and (z == 0).all()
+    eng.close()
+
+
+def test_observables_off_keeps_lds_path_and_errors_when_off():
+    from espins_amd.engine import Engine
+    from espins_amd._lib import EspError
+    m = Lt.cubic_model("sc", 4, J=(1.0,))
+    eng = Engine(0)
+    eng.set_model(m)
+    eng.set_replicas([1.0, 2.0], Lt.random_spins(m.N, 2, 1, m.mag))
+    assert eng.model_info()["lds"]
+    with pytest.raises(EspError):
+        eng.obs(1, (2, 3))
+    eng.set_pm(True)
+    eng.run(1, steps_per_round=3, steps_measure=1, measure=True, exchange=False, seed=1)
+    h, n = eng.obs(4, (2, 1, 201))
+    assert n == 3 and h.sum() == 3 * 2 * m.N        # one count per cell and measurement
+    eng.set_pm(False)
+    with pytest.raises(EspError):
+        eng.obs(4, (2, 1, 201))
+    eng.close()
