@@ -15,7 +15,12 @@ module espins_hip
             esp_set_supercell, esp_set_model, esp_get_model_info, esp_set_replicas, &
             esp_get_spins, esp_put_spins, esp_energy, esp_run, esp_get_stats, esp_get_map, &
             esp_reset_stats, esp_sync, esp_error_text, esp_sweep, esp_learn_set_weights, &
-            esp_learn_collect, esp_learn_train, esp_learn_clear, esp_set_trace, esp_get_trace
+            esp_learn_collect, esp_learn_train, esp_learn_clear, esp_set_trace, esp_get_trace, &
+            esp_set_orderparam, esp_set_staggered, esp_set_pm, esp_set_spincorr, esp_set_sfactor, &
+            esp_get_obs, esp_get_obs_trace
+
+  integer(c_int), parameter, public :: ESP_OBS_ORDER = 1, ESP_OBS_STAGGERED = 2, ESP_OBS_PM = 4, &
+                                       ESP_OBS_SPINCORR = 8, ESP_OBS_SFACTOR = 16
 
   interface
     integer(c_int) function esp_create(ctx, device) bind(C, name='esp_create')
@@ -129,6 +134,50 @@ module espins_hip
       integer(c_int), value :: capacity
     end function
     integer(c_int) function esp_get_trace(ctx, trace, n) bind(C, name='esp_get_trace')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: trace(*)
+      integer(c_int), intent(out) :: n
+    end function
+    ! optional observables
+    integer(c_int) function esp_set_orderparam(ctx, axes) bind(C, name='esp_set_orderparam')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: axes(*)
+    end function
+    integer(c_int) function esp_set_staggered(ctx, enable, coeff) bind(C, name='esp_set_staggered')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable
+      real(c_double), intent(in) :: coeff(*)
+    end function
+    integer(c_int) function esp_set_pm(ctx, enable, nfreq) bind(C, name='esp_set_pm')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable
+      integer(c_int), intent(out) :: nfreq
+    end function
+    integer(c_int) function esp_set_spincorr(ctx, nshell, nspecies, species, zs, cnt, nbr) &
+        bind(C, name='esp_set_spincorr')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nshell, nspecies, zs
+      integer(c_int), intent(in) :: species(*), cnt(*), nbr(*)
+    end function
+    integer(c_int) function esp_set_sfactor(ctx, nq, q, pos, polar, every) bind(C, name='esp_set_sfactor')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nq, every
+      real(c_double), intent(in) :: q(*), pos(*), polar(*)
+    end function
+    integer(c_int) function esp_get_obs(ctx, which, out, count) bind(C, name='esp_get_obs')
+      import :: c_int, c_double, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: which
+      real(c_double), intent(out) :: out(*)
+      integer(c_int64_t), intent(out) :: count
+    end function
+    integer(c_int) function esp_get_obs_trace(ctx, trace, n) bind(C, name='esp_get_obs_trace')
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: trace(*)

@@ -35,6 +35,8 @@ program mc_gpu
   type(mt_t) :: rng
   integer :: rank_seed
   real(c_double), allocatable :: e_init(:), e_slot(:), d_slot(:)
+  integer(c_int) :: nfreq
+  integer :: nshell
 
   narg = command_argument_count()
   if (narg < 1) then
@@ -127,6 +129,7 @@ program mc_gpu
     call read_sconfig(trim(seedname)//'_sconfig.dat')
   end if
   call check(esp_set_replicas(ctx, int(R, c_int), tems, spins), 'esp_set_replicas')
+  call setup_observables()
   call check(esp_energy(ctx, e0), 'esp_energy')
   mode = merge(ESP_MODE_CONST, ESP_MODE_RANDOM, inp%mcarlo_mode(1:5) == 'const')
   learned = inp%mcarlo_mode(1:5) == 'learn'   ! warm-up with 'random', then train, then learned
@@ -143,11 +146,13 @@ program mc_gpu
     if (learned) call train_proposal()
     call check(esp_get_stats(ctx, acc, rej_w, erun, nmeas), 'esp_get_stats')
     call check(esp_reset_stats(ctx), 'esp_reset_stats')
-    if (inp%energy_write) call check(esp_set_trace(ctx, int(inp%steps_mc, c_int)), 'esp_set_trace')
+    if (inp%energy_write .or. inp%lbinerror) &
+      call check(esp_set_trace(ctx, int(inp%steps_mc, c_int)), 'esp_set_trace')
     call check(esp_run(ctx, 1_c_int, int(inp%steps_mc, c_int), int(inp%steps_measure, c_int), &
                1_c_int, 0_c_int, 1_c_int, mode, real(inp%tilt, c_double), seed, sweep, 0_c_int64_t, &
                0_c_int), 'esp_run measure')
     steps_mc = inp%steps_mc
+    call write_observables(.false., steps_mc)
     if (inp%energy_write) call write_energy_series(steps_mc)
     call check(esp_get_stats(ctx, acc, rej, erun, nmeas), 'esp_get_stats')
     call check(esp_energy(ctx, edir), 'esp_energy')
@@ -236,7 +241,7 @@ program mc_gpu
       e_init(k) = erun(smap(k))   ! running energy of the configuration at each slot
     end do
     call check(esp_reset_stats(ctx), 'esp_reset_stats')
-    if (inp%energy_write) call check(esp_set_trace(ctx, int(steps_mc, c_int)), 'esp_set_trace')
+    if (inp%energy_write .or. inp%lbinerror) call check(esp_set_trace(ctx, int(steps_mc, c_int)), 'esp_set_trace')
     ! main phase: measurement after every MC step, exchange every pt_steps_swap steps,
     ! parity restarts at 1 (pt.F90:773-908)
     write(out_u, '(/1x,85a)') '*------------------------------ SWAP in Monte-Carlo --------------------------------*'
@@ -246,6 +251,7 @@ program mc_gpu
     call pt_rounds(nsw, steps_mc, 1_c_int, nsw_warm, 'main')
     write(out_u, *) 'Final configuration = ', smap
     write(out_u, '(1x,85a)') '*-----------------------------------------------------------------------------------*'
+    call write_observables(.true., steps_mc)
     if (inp%energy_write) call write_energy_series(steps_mc)
     call check(esp_get_stats(ctx, acc, rej, erun, nmeas), 'esp_get_stats')
     call check(esp_get_map(ctx, smap, nswap), 'esp_get_map')
@@ -266,6 +272,451 @@ program mc_gpu
 224 format(3x,f9.4,1x,i14,1x,i14,5x,i14,7x,f10.8,4x,a)   ! pt.F90:1126
 
 contains
+
+  ! Optional observables on the device (esp_set_orderparam / _staggered / _pm / _spincorr /
+  ! _sfactor) from the input's flags; P(m) is always on, as the reference always writes _pm.dat.
+  subroutine setup_observables()
+    integer(c_int), allocatable :: kinds(:), scnt(:,:), snbr(:,:,:), c1(:), n1(:)
+    real(c_double), allocatable :: f1(:,:), f2(:,:), v(:), val(:), pos(:,:), q(:,:)
+    integer(c_int) :: z, zs, m, st, nq
+    integer :: sh, i, j, x0, x1, x2, a, lx, ly
+    if (inp%lorderparam) call check(esp_set_orderparam(ctx, inp%op_axes), 'esp_set_orderparam')
+    if (inp%lstaggered) call check(esp_set_staggered(ctx, 1_c_int, inp%stag_coeff), 'esp_set_staggered')
+    call check(esp_set_pm(ctx, 1_c_int, nfreq), 'esp_set_pm')
+    if (inp%lspincorr) then
+      ! shell tables: the J rows of each shell expanded alone (shell_neighbors_get,
+      ! jij.F90:449-503, reads jij_get's slots labelled by the rows' sh=)
+      nshell = maxval(inp%J%shell)
+      zs = 1
+      allocate(scnt(N, nshell), snbr(N, inp%J%n, nshell))
+      scnt = 0
+      snbr = 0
+      do sh = 1, nshell
+        m = count(inp%J%shell == sh)
+        if (m == 0) cycle
+        allocate(f1(3, m), f2(3, m), v(m), c1(N), n1(N * m), val(N * m))
+        j = 0
+        do i = 1, inp%J%n
+          if (inp%J%shell(i) /= sh) cycle
+          j = j + 1
+          f1(:, j) = inp%J%f1(:, i)
+          f2(:, j) = inp%J%f2(:, i)
+        end do
+        v = 1
+        st = esp_build_table(int(inp%nb, c_int), inp%basis, inp%moment, inp%L, &
+                             merge(1_c_int, 0_c_int, inp%open_boundary), m, f1, f2, v, c_null_ptr, 1_c_int, &
+                             m, c1, n1, val, z)
+        if (st /= ESP_OK) call input_error('Unrecognised Atomic Position found in jij_parameters')
+        scnt(:, sh) = c1
+        do j = 1, z
+          snbr(:, j, sh) = n1((j - 1) * N + 1:j * N)
+        end do
+        zs = max(zs, z)
+        deallocate(f1, f2, v, c1, n1, val)
+      end do
+      allocate(kinds(N))
+      do i = 1, N
+        kinds(i) = inp%kind(mod(i - 1, inp%nb) + 1) - 1
+      end do
+      call check(esp_set_spincorr(ctx, int(nshell, c_int), int(inp%nsp, c_int), kinds, zs, scnt, &
+                                  pack_nbr(snbr, zs)), 'esp_set_spincorr')
+    end if
+    if (inp%lsfactor) then
+      ! q-points of the 2-D mesh (mcarlo_structure_factor, get_quant.F90:671-681) and the
+      ! Cartesian site positions (mcarlo_get_atoms_pos)
+      nq = (inp%sf_mesh(1) + 1) * (inp%sf_mesh(2) + 1)
+      allocate(q(3, nq), pos(3, N))
+      do i = 0, nq - 1
+        lx = i / (inp%sf_mesh(2) + 1)
+        ly = i - lx * (inp%sf_mesh(2) + 1)
+        q(:, i + 1) = inp%sf_corner + lx / real(inp%sf_mesh(1), dp) * inp%sf_q1 + &
+                      ly / real(inp%sf_mesh(2), dp) * inp%sf_q2
+      end do
+      i = 0
+      do x2 = 0, inp%L(3) - 1
+        do x1 = 0, inp%L(2) - 1
+          do x0 = 0, inp%L(1) - 1
+            do a = 1, inp%nb
+              i = i + 1
+              pos(:, i) = matmul(inp%lat, [real(x0, dp), real(x1, dp), real(x2, dp)] + inp%basis(:, a))
+            end do
+          end do
+        end do
+      end do
+      call check(esp_set_sfactor(ctx, nq, q, pos, inp%sf_polar, int(inp%sf_steps_measure, c_int)), &
+                 'esp_set_sfactor')
+    end if
+  end subroutine
+
+  ! nbr(N, z, shell) with the realised width zs (the C layout [shell][zs][N])
+  function pack_nbr(nb3, zs) result(out)
+    integer(c_int), intent(in) :: nb3(:,:,:)
+    integer(c_int), intent(in) :: zs
+    integer(c_int), allocatable :: out(:)
+    integer :: sh, l
+    allocate(out(size(nb3, 1) * zs * size(nb3, 3)))
+    do sh = 1, size(nb3, 3)
+      do l = 1, zs
+        out(((sh - 1) * zs + (l - 1)) * size(nb3, 1) + 1:((sh - 1) * zs + l) * size(nb3, 1)) = nb3(:, l, sh)
+      end do
+    end do
+  end function
+
+  ! _pm.dat, _op.dat, _staggered.dat, _spincorr.dat, _binerror.dat, _sf-TNNN.dat of the
+  ! measured phase (mcarlo.F90:713-820, pt.F90:1000-1110), values by temperature slot.
+  subroutine write_observables(pt, nm)
+    logical, intent(in) :: pt
+    integer, intent(in) :: nm
+    real(c_double), allocatable :: pm(:,:,:), ob(:,:), sc(:,:,:,:,:), sf(:,:,:), tr(:,:,:), otr(:,:,:)
+    real(dp), allocatable :: ser(:,:), var(:,:)
+    integer(c_int64_t) :: cnt
+    integer(c_int) :: nrec
+    integer :: u, t, k, nd, ns, l, a, b, nq, nk
+    real(dp) :: nrm, mmax, dmm
+    nrm = 1.0_dp / real(nm, dp)
+    ! ---- P(m) (Pm_get, get_quant.F90:425-493; normalisation mcarlo.F90:715) ----------------
+    nd = 1 + merge(1, 0, inp%lorderparam) + merge(1, 0, inp%lstaggered)
+    allocate(pm(nfreq, nd, R))
+    call check(esp_get_obs(ctx, ESP_OBS_PM, pm, cnt), 'esp_get_obs')
+    mmax = sum(abs(inp%moment))
+    dmm = 2 * mmax / real(nfreq - 1, dp)
+    open(newunit=u, file=trim(seedname)//'_pm.dat', status='replace', action='write')
+    write(u, *) '#written on '//stamp()
+    call pm_header(u, pt)
+    do t = 1, R
+      write(u, '(1x,a,2x,f9.4)') '#T=', tems(t)
+      do k = 1, nfreq
+        write(u, '(2x,E15.8,4(10x,E15.8))') -mmax + real(k - 1, dp) * dmm, &
+          (pm(k, a, t) * (1.0_dp / 0.01_dp) * nrm / real(N / inp%nb, dp), a = 1, nd)
+      end do
+    end do
+    close(u)
+    ! ---- order parameter / staggered moment (mcarlo.F90:773-784) ---------------------------
+    allocate(ob(3, R))
+    if (inp%lorderparam) then
+      call check(esp_get_obs(ctx, ESP_OBS_ORDER, ob, cnt), 'esp_get_obs')
+      call moment_file('_op.dat', '#------------------------------- MONTECARLO -----------------------------------#', &
+                       '#    Temp              OP                 Sus_OP                 U_OP          #', &
+                       '#------------------------------------------------------------------------------#', ob, nrm, pt)
+    end if
+    if (inp%lstaggered) then
+      call check(esp_get_obs(ctx, ESP_OBS_STAGGERED, ob, cnt), 'esp_get_obs')
+      call moment_file('_staggered.dat', '#------------------------------- MONTECARLO --------------------------------#', &
+                       '#    Temp         Staggered_m          Sus_Staggered_m       U_Staggered_m  #', &
+                       '#---------------------------------------------------------------------------#', ob, nrm, pt)
+    end if
+    ! ---- spin correlations (mcarlo.F90:753-764) ---------------------------------------------
+    if (inp%lspincorr) then
+      ns = inp%nsp
+      allocate(sc(ns, ns, nshell, 3, R))
+      call check(esp_get_obs(ctx, ESP_OBS_SPINCORR, sc, cnt), 'esp_get_obs')
+      open(newunit=u, file=trim(seedname)//'_spincorr.dat', status='replace', action='write')
+      write(u, *) '#written on '//stamp()
+      write(u, '(1x,a)') '#---------------------------------------------------- MONTECARLO '// &
+                         '----------------------------------------------------#'
+      write(u, '(1x,a)') '#  Temp    Shell   Atom1   Type1   Atom2   Type2   '// &
+                         '<\sum_{ij} Si.Sj/N>   <\sum_{ij}|Si.Sj|/N>   <|\sum_{ij}Si.Sj|/N> #'
+      write(u, '(1x,a)') '#----------------------------------------------------------------'// &
+                         '----------------------------------------------------#'
+      do t = 1, R
+        do l = 1, nshell
+          do a = 1, ns
+            do b = 1, ns
+              write(u, '(f9.4,1x,I5,6x,a3,4x,i3,6x,a3,4x,i3,8x,f12.9,10x,f12.9,12x,f12.9)') tems(t), l, &
+                symbol(a), a, symbol(b), b, sc(b, a, l, 1, t) * nrm / N, sc(b, a, l, 3, t) * nrm / N, &
+                sc(b, a, l, 2, t) * nrm / N
+            end do
+          end do
+        end do
+      end do
+      close(u)
+    end if
+    ! ---- binning errors (mcarlo.F90:548-590,635-656; pt.F90:840-932) ------------------------
+    if (inp%lbinerror) then
+      allocate(tr(2, R, nm))
+      call check(esp_get_trace(ctx, tr, nrec), 'esp_get_trace')
+      nk = 2 + merge(1, 0, inp%lorderparam) + merge(1, 0, inp%lstaggered)
+      if (nk > 2) then
+        allocate(otr(2, R, nm))
+        call check(esp_get_obs_trace(ctx, otr, nrec), 'esp_get_obs_trace')
+      end if
+      open(newunit=u, file=trim(seedname)//'_binerror.dat', status='replace', action='write')
+      write(u, *) '#written on '//stamp()
+      call binerror_header(u, pt)
+      allocate(ser(nm, nk), var(nk, size(inp%binning_level)))
+      do t = 1, R
+        ser(:, 1) = tr(1, t, 1:nm) / real(N, dp)
+        ser(:, 2) = tr(2, t, 1:nm)
+        k = 2
+        if (inp%lorderparam) then
+          k = k + 1
+          ser(:, k) = otr(1, t, 1:nm)
+        end if
+        if (inp%lstaggered) then
+          k = k + 1
+          ser(:, k) = otr(2, t, 1:nm)
+        end if
+        call binning(ser, inp%binning_level, pt, var)
+        if (pt) then
+          write(u, '(1x,a,2x,f6.2)') '#T=', tems(t)
+        else
+          write(u, '(1x,a,2x,f9.4)') '#T=', tems(t)
+        end if
+        do l = 1, size(inp%binning_level)
+          if (nk == 4 .and. .not. pt) then
+            write(u, '(2x,I3,8x,E15.8,3(3x,E15.8))') inp%binning_level(l), (sqrt(var(a, l) / N), a = 1, nk)
+          else
+            write(u, '(2x,I8,8x,E15.8,3(3x,E15.8))') inp%binning_level(l), (sqrt(var(a, l) / N), a = 1, nk)
+          end if
+        end do
+      end do
+      close(u)
+    end if
+    ! ---- structure factor (get_quant.F90:719-920) -----------------------------------------
+    if (inp%lsfactor) then
+      nq = (inp%sf_mesh(1) + 1) * (inp%sf_mesh(2) + 1)
+      allocate(sf(nq, 3, R))
+      call check(esp_get_obs(ctx, ESP_OBS_SFACTOR, sf, cnt), 'esp_get_obs')
+      call write_sfactor(sf, cnt)
+    end if
+  end subroutine
+
+  character(len=24) function stamp()
+    character(len=8) :: d
+    character(len=10) :: tm
+    call date_and_time(date=d, time=tm)
+    stamp = d(7:8)//'/'//d(5:6)//'/'//d(1:4)//' at '//tm(1:2)//':'//tm(3:4)//':'//tm(5:6)
+  end function
+
+  ! the reference's atoms_symbol: the species label with an upper-case first letter
+  character(len=3) function symbol(k)
+    integer, intent(in) :: k
+    integer :: c
+    symbol = inp%species(k)
+    c = ichar(symbol(1:1))
+    if (c >= ichar('a') .and. c <= ichar('z')) symbol(1:1) = char(c - 32)
+  end function
+
+  ! _op.dat / _staggered.dat: <|X|>, susceptibility, Binder cumulant per temperature
+  ! (mcarlo.F90:773-784, format 126 of mcarlo.F90:855 / pt.F90:1128)
+  subroutine moment_file(suffix, h1, h2, h3, ob, nrm, pt)
+    character(len=*), intent(in) :: suffix, h1, h2, h3
+    real(c_double), intent(in) :: ob(:,:)
+    real(dp), intent(in) :: nrm
+    logical, intent(in) :: pt
+    integer :: uu, tt
+    real(dp) :: bt, x1, x2, x4
+    open(newunit=uu, file=trim(seedname)//suffix, status='replace', action='write')
+    write(uu, *) '#written on '//stamp()
+    write(uu, '(1x,a)') h1
+    write(uu, '(1x,a)') h2
+    write(uu, '(1x,a)') h3
+    do tt = 1, R
+      bt = 1.0_dp / tems(tt)
+      x1 = ob(1, tt) * nrm
+      x2 = ob(2, tt) * nrm
+      x4 = ob(3, tt) * nrm
+      if (pt) then
+        write(uu, '(3x,f9.4,5x,E15.8,6x,E15.8,8x,E15.8,5x)') tems(tt), x1, bt * (x2 - x1**2) * N, &
+                                                             1.0_dp - (1.0_dp / 3.0_dp) * (x4 / x2**2)
+      else
+        write(uu, '(3x,f9.4,5x,E15.8,6x,E15.8,7x,E15.8,5x)') tems(tt), x1, bt * (x2 - x1**2) * N, &
+                                                             1.0_dp - (1.0_dp / 3.0_dp) * (x4 / x2**2)
+      end if
+    end do
+    close(uu)
+  end subroutine
+
+  ! table headers of _pm.dat and _binerror.dat (mcarlo.F90:172-215, pt.F90:159-200)
+  subroutine pm_header(u, pt)
+    integer, intent(in) :: u
+    logical, intent(in) :: pt
+    if (inp%lorderparam .and. inp%lstaggered) then
+      if (pt) then
+        write(u, '(1x,a)') '#-------------------------------------- MONTECARLO ------'//'------------------------------------#'
+        write(u, '(1x,a)') '#       M or OP                 P(M)                     '//'P(OP)               P(Staggered_m)  #'
+        write(u, '(1x,a)') '#--------------------------------------------------------'//'------------------------------------#'
+      else
+        write(u, '(1x,a)') '#------------------------------------- MONTECARLO ---------'//'----------------------------------#'
+        write(u, '(1x,a)') '#       M or OP                 P(M)                     P('//'OP)               P(Staggered_m)  #'
+        write(u, '(1x,a)') '#----------------------------------------------------------'//'----------------------------------#'
+      end if
+    else if (inp%lorderparam) then
+      if (pt) then
+        write(u, '(1x,a)') '#---------------------------- MONTECARLO -----------------------------#'
+      else
+        write(u, '(1x,a)') '#-------------------------- MONTECARLO -------------------------------#'
+      end if
+      write(u, '(1x,a)') '#       M or OP                 P(M)                     P(OP)        #'
+      write(u, '(1x,a)') '#---------------------------------------------------------------------#'
+    else if (inp%lstaggered) then
+      if (pt) then
+        write(u, '(1x,a)') '#--------------------------- MONTECARLO-----------------------------#'
+      else
+        write(u, '(1x,a)') '#-------------------------- MONTECARLO -----------------------------#'
+      end if
+      write(u, '(1x,a)') '# M or Staggered                P(M)                P(Staggered_m)  #'
+      write(u, '(1x,a)') '#-------------------------------------------------------------------#'
+    else
+      write(u, '(1x,a)') '#----------------- MONTECARLO -------------------#'
+      write(u, '(1x,a)') '#        M                      P(M)             #'
+      write(u, '(1x,a)') '#------------------------------------------------#'
+    end if
+  end subroutine
+
+  subroutine binerror_header(u, pt)
+    integer, intent(in) :: u
+    logical, intent(in) :: pt
+    if (inp%lorderparam .and. inp%lstaggered) then
+      if (pt) then
+        write(u, '(1x,a)') '#------------------------------------- MONTECARLO ---------------------'//'--------------------#'
+        write(u, '(1x,a)') '#  Binning_level     Error(E)          Error(M)        Error(OP)       '//'Error(Staggered_m)  #'
+        write(u, '(1x,a)') '#----------------------------------------------------------------------'//'--------------------#'
+      else
+        write(u, '(1x,a)') '#------------------------------------ MONTECARLO ------'//'------------------------------------#'
+        write(u, '(1x,a)') '#  Binning_level     Error(E)          Error(M)        '//'Error(OP)       Error(Staggered_m)  #'
+        write(u, '(1x,a)') '#------------------------------------------------------'//' ------------------------------------#'
+      end if
+    else if (inp%lorderparam) then
+      if (pt) then
+        write(u, '(1x,a)') '#--------------------------- MONTECARLO -----------------------------#'
+      else
+        write(u, '(1x,a)') '#------------------------- MONTECARLO -------------------------------#'
+      end if
+      write(u, '(1x,a)') '#  Binning_level     Error(E)          Error(M)         Error(OP)    #'
+      write(u, '(1x,a)') '#--------------------------------------------------------------------#'
+    else if (inp%lstaggered) then
+      write(u, '(1x,a)') '#----------------------------- MONTECARLO -------------------------------#'
+      write(u, '(1x,a)') '#  Binning_level     Error(E)          Error(M)      Error(Staggered_m)  #'
+      write(u, '(1x,a)') '#------------------------------------------------------------------------#'
+    else
+      write(u, '(1x,a)') '#------------------- MONTECARLO -----------------#'
+      write(u, '(1x,a)') '#  Block_length      Error(E)          Error(M)  #'
+      write(u, '(1x,a)') '#------------------------------------------------#'
+    end if
+  end subroutine
+
+  ! Binning analysis of one temperature's measurement series ser(n, k): box means of 2^level
+  ! measurements, their spread, the incomplete last box left out (mcarlo.F90:582-590,635-656).
+  ! The parallel-tempering loop divides a completed box by 2^(index of the level) instead of
+  ! 2^level (pt.F90:843); with the default levels 1..5 both agree.
+  subroutine binning(ser, levels, pt, var)
+    real(dp), intent(in) :: ser(:,:)
+    integer, intent(in) :: levels(:)
+    logical, intent(in) :: pt
+    real(dp), intent(out) :: var(:,:)
+    real(dp) :: box(size(ser, 2), size(levels)), acc2(size(ser, 2), size(levels)), tot(size(ser, 2))
+    integer :: n, st, l, ml
+    n = size(ser, 1)
+    box = 0
+    acc2 = 0
+    tot = 0
+    do st = 1, n
+      tot = tot + ser(st, :)
+      do l = 1, size(levels)
+        box(:, l) = box(:, l) + ser(st, :)
+        if (mod(st, 2**levels(l)) == 0) then
+          if (pt) then
+            box(:, l) = box(:, l) / real(2**l, dp)
+          else
+            box(:, l) = box(:, l) / real(2**levels(l), dp)
+          end if
+          acc2(:, l) = acc2(:, l) + box(:, l)**2
+          box(:, l) = 0
+        end if
+      end do
+    end do
+    do l = 1, size(levels)
+      ml = n / 2**levels(l)
+      box(:, l) = (tot - box(:, l)) / real(2**levels(l), dp) / real(ml, dp)
+      var(:, l) = (acc2(:, l) / real(ml, dp) - box(:, l)**2) / real(ml - 1, dp)
+    end do
+  end subroutine
+
+  ! seedname_sf-TNNN.dat per temperature: in-plane coordinates of every q-point of the mesh
+  ! and S(q) per site and measurement (mcarlo_structure_factor_write_mc, get_quant.F90:719-820),
+  ! plus a short plotting script.
+  subroutine write_sfactor(sf, cnt)
+    real(c_double), intent(in) :: sf(:,:,:)
+    integer(c_int64_t), intent(in) :: cnt
+    real(dp), parameter :: twopi = 6.283185307179586476925286766559_dp
+    real(dp) :: qv(3,3), av(3,3), zv(3), yv(3), q1m, q2m, ym, cq, cy, c1, c2, qx, qy, f1, f2, e(3)
+    integer :: u, t, i, lx, ly, k
+    character(len=300) :: fn
+    q1m = sqrt(sum(inp%sf_q1**2))
+    q2m = sqrt(sum(inp%sf_q2**2))
+    zv = cross(inp%sf_q1, inp%sf_q2)
+    yv = cross(zv, inp%sf_q1)
+    ym = sqrt(sum(yv**2))
+    qv(:, 1) = inp%sf_q1
+    qv(:, 2) = inp%sf_q2
+    qv(:, 3) = zv / sqrt(sum(zv**2))
+    ! dual vectors: av(:, k) . qv(:, j) = 2 pi delta_jk
+    do k = 1, 3
+      e = 0
+      e(k) = 1
+      block
+        real(dp) :: f(3)
+        call solve3(qv, e, f)
+        av(k, :) = twopi * f
+      end block
+    end do
+    cy = dot_product(yv, inp%sf_q2) / (ym * q2m)
+    cq = dot_product(inp%sf_q1, inp%sf_q2) / (q1m * q2m)
+    c1 = dot_product(inp%sf_corner, av(:, 1)) / twopi
+    c2 = dot_product(inp%sf_corner, av(:, 2)) / twopi
+    do t = 1, R
+      write(fn, '(a,a,i3.3,a)') trim(seedname), '_sf-T', t, '.dat'
+      write(out_u, '(3x,a,f8.3,3x,a,3x,a)') 'Output Structure factor at T = ', tems(t), 'writted at', trim(fn)
+      open(newunit=u, file=trim(fn), status='replace', action='write')
+      do i = 0, size(sf, 1) - 1
+        lx = i / (inp%sf_mesh(2) + 1)
+        ly = i - lx * (inp%sf_mesh(2) + 1)
+        f1 = lx / real(inp%sf_mesh(1), dp) + c1
+        f2 = ly / real(inp%sf_mesh(2), dp) + c2
+        qx = f1 * q1m + f2 * q2m * cq
+        qy = f2 * q2m * cy
+        if (inp%lsf_polar) then
+          write(u, '(3x,E15.8,3x,E15.8,3x,E15.8,3x,E15.8,3x,E15.8)') qx, qy, &
+            (sf(i + 1, k, t) / N / real(max(cnt, 1_c_int64_t), dp), k = 1, 3)
+        else
+          write(u, '(3x,E15.8,3x,E15.8,3x,E15.8)') qx, qy, sf(i + 1, 1, t) / N / real(max(cnt, 1_c_int64_t), dp)
+        end if
+      end do
+      close(u)
+    end do
+    write(out_u, '(4x,a,I14,17x,a)') ' Total used steps at S Factor calculation = ', cnt, '   <-- SFAC'
+    open(newunit=u, file=trim(seedname)//'_sf.py', status='replace', action='write')
+    write(u, '(a)') '# S(q) of the first temperature on the (q1, q2) plane: columns qx, qy, S'
+    write(u, '(a)') 'import numpy as np, matplotlib.pyplot as plt'
+    write(u, '(a)') "d = np.loadtxt('"//trim(seedname)//"_sf-T001.dat')"
+    write(u, '(a)') 'plt.tricontourf(d[:, 0], d[:, 1], d[:, 2], 200)'
+    write(u, '(a)') "plt.gca().set_aspect('equal'); plt.colorbar()"
+    write(u, '(a)') "plt.savefig('"//trim(seedname)//"_sf-T001.pdf')"
+    close(u)
+  end subroutine
+
+  function cross(a, b) result(c)
+    real(dp), intent(in) :: a(3), b(3)
+    real(dp) :: c(3)
+    c = [a(2) * b(3) - a(3) * b(2), a(3) * b(1) - a(1) * b(3), a(1) * b(2) - a(2) * b(1)]
+  end function
+
+  ! x with A^T x = e  (A's columns are the vectors; x is then a dual vector over 2 pi)
+  subroutine solve3(A, e, x)
+    real(dp), intent(in) :: A(3,3), e(3)
+    real(dp), intent(out) :: x(3)
+    real(dp) :: M(3,3), det
+    M = transpose(A)
+    det = M(1,1)*(M(2,2)*M(3,3)-M(3,2)*M(2,3)) - M(1,2)*(M(2,1)*M(3,3)-M(2,3)*M(3,1)) &
+        + M(1,3)*(M(2,1)*M(3,2)-M(2,2)*M(3,1))
+    x(1) = (e(1)*(M(2,2)*M(3,3)-M(3,2)*M(2,3)) - M(1,2)*(e(2)*M(3,3)-M(2,3)*e(3)) &
+          + M(1,3)*(e(2)*M(3,2)-M(2,2)*e(3))) / det
+    x(2) = (M(1,1)*(e(2)*M(3,3)-e(3)*M(2,3)) - e(1)*(M(2,1)*M(3,3)-M(2,3)*M(3,1)) &
+          + M(1,3)*(M(2,1)*e(3)-e(2)*M(3,1))) / det
+    x(3) = (M(1,1)*(M(2,2)*e(3)-M(3,2)*e(2)) - M(1,2)*(M(2,1)*e(3)-e(2)*M(3,1)) &
+          + e(1)*(M(2,1)*M(3,2)-M(2,2)*M(3,1))) / det
+  end subroutine
 
   ! seedname_energy-TNNN.dat, one file per temperature (open_E_unit / write_E / close_E_unit,
   ! get_quant.F90:1130-1223): header, then the running energy per site of every measured MC

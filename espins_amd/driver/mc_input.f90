@@ -19,6 +19,7 @@ module mc_input
   type, public :: rows_t
     integer :: n = 0
     real(dp), allocatable :: f1(:,:), f2(:,:), val(:), dvec(:,:)
+    integer, allocatable :: shell(:)            ! sh= of each row (0 when absent)
   end type
 
   type, public :: input_t
@@ -42,6 +43,18 @@ module mc_input
     integer :: seed = 0
     logical :: have_seed = .false., open_boundary = .false.
     logical :: energy_write = .false.     ! _energy-TNNN.dat series (parameters.F90:536-537)
+    ! species (in order of first appearance) and the species of each sorted atom
+    integer :: nsp = 0
+    character(len=8), allocatable :: species(:)
+    integer, allocatable :: kind(:)
+    ! optional observables (parameters.F90:440-592)
+    logical :: lorderparam = .false., lstaggered = .false., lspincorr = .false.
+    logical :: lbinerror = .false., lsfactor = .false., lsf_polar = .false.
+    real(dp), allocatable :: op_axes(:,:), stag_coeff(:)
+    integer, allocatable :: binning_level(:)
+    integer :: sf_mesh(2) = 50, sf_steps_measure = 100
+    real(dp) :: sf_corner(3) = 0, sf_q1(3) = 0, sf_q2(3) = 0, sf_polar(3) = 0
+    real(dp) :: recip(3,3) = 0                  ! recip(:,k) = reciprocal vector k (2 pi / Angstrom)
   end type
 
   public :: read_input, input_error
@@ -156,6 +169,67 @@ contains
     call input_error('begin '//trim(name)//' without end')
   end subroutine
 
+  ! reciprocal vectors recip(:,k) with lat(:,j) . recip(:,k) = 2 pi delta_jk
+  subroutine recip_of(lat, recip)
+    real(dp), intent(in) :: lat(3,3)
+    real(dp), intent(out) :: recip(3,3)
+    real(dp), parameter :: twopi = 6.283185307179586476925286766559_dp
+    real(dp) :: e(3), f(3)
+    integer :: k
+    do k = 1, 3
+      e = 0
+      e(k) = 1
+      call cart_to_frac(lat, e, f)        ! f = column k of inv(lat)
+      recip(k, :) = twopi * f             ! recip = 2 pi inv(lat)^T
+    end do
+  end subroutine
+
+  ! Integer lists such as "1-5", "1 3 5" or "2,4-6" (param_get_range_vector, parameters.F90:2245-2341)
+  subroutine range_vector(w, out)
+    character(len=*), intent(in) :: w
+    integer, allocatable, intent(out) :: out(:)
+    character(len=len(w)) :: t
+    integer :: p, q, a, b, k, n, buf(1024), ios
+    t = w
+    do p = 1, len(t)
+      if (t(p:p) == ',' .or. t(p:p) == ';') t(p:p) = ' '
+      if (t(p:p) == ':') t(p:p) = '-'
+    end do
+    n = 0
+    p = 1
+    do
+      do while (p <= len_trim(t))
+        if (t(p:p) /= ' ') exit
+        p = p + 1
+      end do
+      if (p > len_trim(t)) exit
+      q = p
+      do while (q <= len_trim(t))
+        if (t(q:q) == ' ') exit
+        q = q + 1
+      end do
+      k = index(t(p:q-1), '-')
+      if (k > 1) then
+        read(t(p:p+k-2), *, iostat=ios) a
+        if (ios /= 0) call input_error('Error parsing range '//trim(w))
+        read(t(p+k:q-1), *, iostat=ios) b
+        if (ios /= 0) call input_error('Error parsing range '//trim(w))
+        do k = min(a, b), max(a, b)
+          n = n + 1
+          buf(n) = k
+        end do
+      else
+        read(t(p:q-1), *, iostat=ios) a
+        if (ios /= 0) call input_error('Error parsing range '//trim(w))
+        n = n + 1
+        buf(n) = a
+      end if
+      p = q
+    end do
+    allocate(out(n))
+    out = buf(1:n)
+  end subroutine
+
   subroutine cart_to_frac(lat, c, f)
     real(dp), intent(in) :: lat(3,3), c(3)
     real(dp), intent(out) :: f(3)
@@ -199,7 +273,8 @@ contains
       if (len_trim(lines(k)) > 0) n = n + 1
     end do
     r%n = n
-    allocate(r%f1(3,n), r%f2(3,n), r%val(n))
+    allocate(r%f1(3,n), r%f2(3,n), r%val(n), r%shell(n))
+    r%shell = 0
     n = 0
     do k = first, last
       if (len_trim(lines(k)) == 0) cycle
@@ -232,6 +307,12 @@ contains
       read(tok(3)(len(key)+2:), *, iostat=ios) r%val(n)
       if (ios /= 0) call input_error('bad value in '//trim(lines(k)))
       r%val(n) = r%val(n) * econv
+      do p = 4, nt                ! the optional sh= field (param_get_parameters, parameters.F90:2751-2766)
+        if (tok(p)(1:3) == 'sh=') then
+          read(tok(p)(4:), *, iostat=ios) r%shell(n)
+          if (ios /= 0) call input_error('bad sh= in '//trim(lines(k)))
+        end if
+      end do
     end do
   contains
     subroutine site(t, tag, f)
@@ -299,6 +380,7 @@ contains
     character(len=maxl) :: v
     character(len=8), allocatable :: lab(:), species(:)
     real(dp), allocatable :: pos(:,:), mom(:)
+    integer, allocatable :: orig(:)
     real(dp) :: scale, step, x(3)
     real(dp), allocatable :: dv(:,:)
 
@@ -356,7 +438,7 @@ contains
       end if
     end do
     inp%nb = n
-    allocate(inp%label(n), inp%basis(3, n), inp%moment(n))
+    allocate(inp%label(n), inp%basis(3, n), inp%moment(n), inp%kind(n), orig(n))
     a = 0
     do s = 1, nsp
       do k = 1, n
@@ -365,9 +447,15 @@ contains
           inp%label(a) = lab(k)
           inp%basis(:, a) = pos(:, k)
           inp%moment(a) = mom(k)
+          inp%kind(a) = s
+          orig(a) = k
         end if
       end do
     end do
+    inp%nsp = nsp
+    allocate(inp%species(nsp))
+    inp%species = species(1:nsp)
+    call recip_of(inp%lat, inp%recip)
 
     call keyword('supercell_size', found, v)
     if (found) read(v, *) inp%L
@@ -498,32 +586,144 @@ contains
       call read_axes(inp, 'field_axes', inp%h_par, inp%h_axis, found)
       if (.not. found) call input_error('ham_field is true but no field_axes block found')
     end if
+    call observable_keywords()
     call ignored_keywords()
     call unsupported_keywords()
     call unrecognised_keywords(fname)
   contains
+    ! The optional observables (parameters.F90:440-592): flags, axes, coefficients, binning
+    ! levels and the q-mesh of the structure factor, with the reference's checks.
+    subroutine observable_keywords()
+      character(len=maxl) :: w
+      integer :: ic(inp%nb), q
+      logical :: ff, fc
+      real(dp) :: v3(3), cr(3)
+      call keyword('staggered_m', found, w)
+      if (found) inp%lstaggered = is_true(w)
+      if (inp%lstaggered .and. inp%nb == 1) then
+        write(*, '(1x,a)') 'WARNING: Calculation of staggered magnetization for one atom is'
+        write(*, '(9x,a)') ' meaningless staggered_m was changed to .False.'
+        inp%lstaggered = .false.
+      end if
+      allocate(inp%stag_coeff(inp%nb))
+      inp%stag_coeff = 1
+      call keyword('staggered_m_coeff', found, w)
+      if (found) then
+        read(w, *, iostat=ios) ic
+        if (ios /= 0) call input_error('staggered_m_coeff needs one integer per atom')
+        if (inp%lstaggered .and. any(abs(ic) /= 1)) call input_error('staggered_m_coeff must be +1 or -1')
+        do q = 1, inp%nb                ! input atom order -> species-sorted order
+          inp%stag_coeff(q) = ic(orig(q))
+        end do
+      end if
+      call keyword('order_parameter', found, w)
+      if (found) inp%lorderparam = is_true(w)
+      call axes_block(ff, fc)
+      if (ff .and. fc) call input_error('Cannot specify both order_parameter_axes_frac and order_parameter_axes_cart')
+      if (inp%lorderparam .and. .not. (ff .or. fc)) &
+        call input_error('No order_parameter_axes_frac or order_parameter_axes_cart block found')
+      call keyword('binning_error', found, w)
+      if (found) inp%lbinerror = is_true(w)
+      call keyword('binning_level', found, w)
+      if (found) then
+        call range_vector(w, inp%binning_level)
+        if (size(inp%binning_level) < 1) call input_error('problem reading binning_level')
+      else
+        allocate(inp%binning_level(5))
+        inp%binning_level = [1, 2, 3, 4, 5]
+      end if
+      if (inp%lbinerror) then
+        if (any(inp%binning_level <= 0) .or. any(inp%binning_level > 30)) &
+          call input_error('binning_level contain non-valid numbers. It must be greater than zero and as ' &
+                           //'(steps_mc/2^(binning_level))-1>0.')
+        if (any(inp%steps_mc / (2**inp%binning_level) <= 1)) &
+          call input_error('binning_level contain non-valid numbers. It must be greater than zero and as ' &
+                           //'(steps_mc/2^(binning_level))-1>0.')
+      end if
+      call keyword('spin_correlation', found, w)
+      if (found) inp%lspincorr = is_true(w)
+      if (inp%lspincorr .and. any(inp%J%shell <= 0)) &
+        call input_error('sh in jij_parameters block must be greater than zero')
+      call keyword('sfactor', found, w)
+      if (found) inp%lsfactor = is_true(w)
+      v3 = [1.0_dp, 0.0_dp, 0.0_dp]
+      call keyword('sfactor_polar', found, w)
+      if (found) then
+        read(w, *, iostat=ios) v3
+        if (ios /= 0) call input_error('bad sfactor_polar')
+        inp%lsf_polar = .true.
+      end if
+      inp%sf_polar = matmul(inp%recip, v3)
+      v3 = 0
+      call keyword('sfactor_corner', found, w)
+      if (found) read(w, *, iostat=ios) v3
+      inp%sf_corner = matmul(inp%recip, v3)
+      v3 = [1.0_dp, 0.0_dp, 0.0_dp]
+      call keyword('sfactor_q1', found, w)
+      if (found) read(w, *, iostat=ios) v3
+      inp%sf_q1 = matmul(inp%recip, v3)
+      v3 = [0.0_dp, 0.0_dp, 1.0_dp]
+      call keyword('sfactor_q2', found, w)
+      if (found) read(w, *, iostat=ios) v3
+      inp%sf_q2 = matmul(inp%recip, v3)
+      cr = [inp%sf_q1(2)*inp%sf_q2(3) - inp%sf_q1(3)*inp%sf_q2(2), inp%sf_q1(3)*inp%sf_q2(1) - inp%sf_q1(1)*inp%sf_q2(3), &
+            inp%sf_q1(1)*inp%sf_q2(2) - inp%sf_q1(2)*inp%sf_q2(1)]
+      if (sqrt(sum(cr**2)) < 1e-8_dp) call input_error('Vectors sfactor_q1 and sfactor_q2 not linearly independent')
+      call keyword('sfactor_steps_measure', found, w)
+      if (found) read(w, *, iostat=ios) inp%sf_steps_measure
+      if (inp%lsfactor .and. (inp%sf_steps_measure < 1 .or. inp%sf_steps_measure > inp%steps_mc)) &
+        call input_error('sfactor_steps_measure must be greater than zero and less than steps_mc.')
+      call keyword('sfactor_2dqmesh', found, w)
+      if (found) read(w, *, iostat=ios) inp%sf_mesh
+      if (inp%lsfactor .and. any(inp%sf_mesh < 1)) call input_error('sfactor_2dqmesh must be greater than zero')
+    end subroutine
+
+    ! order_parameter_axes_frac / _cart: "label x y z" lines, sorted to the species order,
+    ! normalised (param_get_axes with lsort and lnorm, parameters.F90:3061-3280)
+    subroutine axes_block(ff, fc)
+      logical, intent(out) :: ff, fc
+      integer :: f1, f2, c1, c2, k, a, taken(inp%nb), cnt
+      character(len=8) :: lab
+      real(dp) :: v(3)
+      call block('order_parameter_axes_frac', ff, f1, f2)
+      call block('order_parameter_axes_cart', fc, c1, c2)
+      if (.not. (ff .or. fc) .or. (ff .and. fc)) return
+      if (fc) then
+        f1 = c1
+        f2 = c2
+      end if
+      if (lines(f1)(1:3) == 'ang' .or. lines(f1)(1:4) == 'bohr') f1 = f1 + 1
+      allocate(inp%op_axes(3, inp%nb))
+      taken = 0
+      cnt = 0
+      do k = f1, f2
+        if (len_trim(lines(k)) == 0) cycle
+        read(lines(k), *, iostat=ios) lab, v
+        if (ios /= 0) call input_error('bad line in order_parameter_axes block: '//trim(lines(k)))
+        if (ff) v = matmul(inp%lat, v)
+        v = v / sqrt(sum(v * v))
+        do a = 1, inp%nb
+          if (trim(inp%label(a)) == trim(lab) .and. taken(a) == 0) then
+            taken(a) = 1
+            inp%op_axes(:, a) = v
+            cnt = cnt + 1
+            exit
+          end if
+        end do
+      end do
+      if (cnt /= inp%nb) &
+        call input_error('Wrong number of lines in order_parameter_axes_frac or order_parameter_axes_cart block')
+    end subroutine
+
     ! Reference keywords of features this driver does not run yet: accepted when off, an
     ! error when switched on (never silently ignored).
     subroutine unsupported_keywords()
       character(len=maxl) :: w
-      character(len=24), parameter :: flags(6) = [character(len=24) :: 'order_parameter', 'staggered_m', &
-                                                   'spin_correlation', 'sfactor', 'binning_error', 'spin_glass']
-      character(len=24), parameter :: values(9) = [character(len=24) :: 'staggered_m_coeff', 'sfactor_2dqmesh', &
-                                                   'sfactor_corner', 'sfactor_polar', 'sfactor_q1', 'sfactor_q2', &
-                                                   'sfactor_steps_measure', 'binning_level', 'spin_glass_seed']
-      integer :: q, f1, f2
-      logical :: fb
-      do q = 1, size(flags)
-        call keyword(trim(flags(q)), found, w)
-        if (found) then
-          if (is_true(w)) call input_error(trim(flags(q))//' = true is not supported by mc_gpu.x')
-        end if
-      end do
-      do q = 1, size(values)
-        call keyword(trim(values(q)), found, w)
-      end do
-      call block('order_parameter_axes_cart', fb, f1, f2)
-      call block('order_parameter_axes_frac', fb, f1, f2)
+      call keyword('spin_glass', found, w)
+      if (found) then
+        if (is_true(w)) call input_error('spin_glass = true is not supported by mc_gpu.x')
+      end if
+      call keyword('spin_glass_seed', found, w)
     end subroutine
 
     ! Keywords of the reference that only its input-generation modes (-inp1/-inp2/-ham) read,

@@ -31,3 +31,24 @@ def test_reference_setup_keywords_are_accepted_and_checked(tmp_path):
     text = open(os.path.join(GOLDEN, "examples", "ex01_cubic.mcin")).read()
     r = _run(tmp_path, text + "\n length_unit = parsec\n")
     assert r.returncode != 0 and "length_unit not recognised" in r.stdout
+
+
+def _obs_input(**kw):
+    text = open(os.path.join(GOLDEN, "examples", "ex01_cubic.mcin")).read()
+    return text + "\n" + "\n".join(f" {k} = {v}" for k, v in kw.items()) + "\n"
+
+
+def test_observable_keyword_checks(tmp_path):
+    """The reference's checks of the observable keywords (parameters.F90:440-592)."""
+    r = _run(tmp_path, _obs_input(order_parameter=".true."))
+    assert r.returncode != 0 and "No order_parameter_axes_frac or order_parameter_axes_cart" in r.stdout
+    r = _run(tmp_path, _obs_input(binning_error=".true.", binning_level="1-40"))
+    assert r.returncode != 0 and "binning_level contain non-valid numbers" in r.stdout
+    r = _run(tmp_path, _obs_input(sfactor=".true.", sfactor_steps_measure=0))
+    assert r.returncode != 0 and "sfactor_steps_measure must be greater than zero" in r.stdout
+    r = _run(tmp_path, _obs_input(sfactor=".true.", sfactor_2dqmesh="0 4"))
+    assert r.returncode != 0 and "sfactor_2dqmesh must be greater than zero" in r.stdout
+    r = _run(tmp_path, _obs_input(sfactor_q1="1 0 0", sfactor_q2="2 0 0"))
+    assert r.returncode != 0 and "not linearly independent" in r.stdout
+    r = _run(tmp_path, _obs_input(spin_glass=".true."))
+    assert r.returncode != 0 and "spin_glass = true is not supported" in r.stdout

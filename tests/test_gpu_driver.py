@@ -179,3 +179,154 @@ def test_pt_warmup_outputs_and_map_prints(tmp_path):
         rows = [l.split() for l in out.splitlines() if l.rstrip().endswith("<-- " + tag)]
         e = np.array([[float(r[2]), float(r[3])] for r in rows])
         assert np.allclose(e[:, 0], e[:, 1], rtol=1e-6)
+
+
+def _two_kind_bcc(tmp_path, seed, extra):
+    """BCC-type cell with two kinds (A at the corner, B at the centre), cell edge 2 A, J1 between
+    kinds (8 bonds, sh = 1), J2 within a kind along the axes (6 bonds, sh = 2)."""
+    import itertools
+    rows = []
+    for a, fa in (("a", (0.0, 0.0, 0.0)), ("b", (0.5, 0.5, 0.5))):
+        other = (0.5, 0.5, 0.5) if a == "a" else (0.0, 0.0, 0.0)
+        for sx, sy, sz in itertools.product((-0.5, 0.5), repeat=3):
+            f2 = (fa[0] + sx, fa[1] + sy, fa[2] + sz)
+            rows.append(f"f1={fa[0]},{fa[1]},{fa[2]}:f2={f2[0]},{f2[1]},{f2[2]}:jij=-0.002:sh=1")
+        for ax in range(3):
+            for sg in (-1.0, 1.0):
+                f2 = list(fa)
+                f2[ax] += sg
+                rows.append(f"f1={fa[0]},{fa[1]},{fa[2]}:f2={f2[0]},{f2[1]},{f2[2]}:jij=0.0005:sh=2")
+    text = f"""
+ Begin Unit_Cell_Cart
+   2.0 0.0 0.0
+   0.0 2.0 0.0
+   0.0 0.0 2.0
+ End Unit_Cell_Cart
+ Begin Atoms_Frac
+  Fe 0.0 0.0 0.0 1.0
+  Co 0.5 0.5 0.5 1.0
+ End Atoms_Frac
+ tems_mode = lin
+ tem_start = 10
+ tem_end = 40
+ tems_num = 2
+ supercell_size = 4 4 4
+ initial_sconfig = rand
+ seeds = {seed}
+ Begin Jij_parameters
+{chr(10).join(rows)}
+ End Jij_parameters
+{extra}
+"""
+    (tmp_path / "bcc2.mcin").write_text(text)
+
+
+def _sconfig_spins(path, N, R):
+    lines = path.read_text().splitlines()
+    out, k = [], 3
+    for t in range(R):
+        k += 1                                     # temperature line
+        out.append(np.array([[float(v) for v in l.split()] for l in lines[k:k + N]]))
+        k += N
+    return out
+
+
+def test_observable_outputs_one_measurement(tmp_path):
+    """One measured step (steps_mc = 1): every observable file holds the observable of the final
+    configuration, which _sconfig.dat prints (3F12.6), compared with the oracle on those spins."""
+    from espins_amd import lattice as Lt
+    from oracle import obs_oracle as B
+    from oracle import oracle as O
+    extra = """
+ steps_warmup = 30
+ steps_mc = 1
+ steps_measure = 1
+ order_parameter = .true.
+ Begin order_parameter_axes_cart
+  Fe 0 0 1
+  Co 0 0 -1
+ End order_parameter_axes_cart
+ staggered_m = .true.
+ staggered_m_coeff = 1 -1
+ spin_correlation = .true.
+ sfactor = .true.
+ sfactor_2dqmesh = 4 4
+ sfactor_steps_measure = 1
+ sfactor_polar = 0 0 1
+"""
+    _two_kind_bcc(tmp_path, 17, extra)
+    _run(tmp_path, "bcc2")
+    N, R = 128, 2
+    spins = _sconfig_spins(tmp_path / "bcc2_sconfig.dat", N, R)
+    axes = [(0, 0, 1), (0, 0, -1)]
+    coeff = [1.0, -1.0]
+    op = np.loadtxt(tmp_path / "bcc2_op.dat", comments="#")
+    sm = np.loadtxt(tmp_path / "bcc2_staggered.dat", comments="#")
+    # 3F12.6 spins: observables agree to ~1e-5
+    for t in range(R):
+        assert op[t, 1] == pytest.approx(abs(B.orderparam(spins[t], 2, axes)), abs=2e-5)
+        assert sm[t, 1] == pytest.approx(B.staggered(spins[t], 2, coeff), abs=2e-5)
+    # P(m): one measurement, every cell counted once; sum P dm = 1 per column
+    pm_lines = (tmp_path / "bcc2_pm.dat").read_text().splitlines()
+    blocks = [i for i, l in enumerate(pm_lines) if l.strip().startswith("#T=")]
+    assert len(blocks) == R
+    for t, b in enumerate(blocks):
+        rows = np.array([[float(v) for v in l.split()] for l in pm_lines[b + 1:b + 402]])
+        assert rows.shape == (401, 4)
+        assert np.allclose(rows[:, 1:].sum(axis=0) * 0.01, 1.0, rtol=1e-7)
+        h = B.pm_counts(spins[t], 2, [1.0, 1.0], axes=axes, coeff=coeff)
+        got = np.rint(rows[:, 1:].T * 0.01 * (N // 2)).astype(int)
+        # bins agree except where a 3F12.6-rounded value moved across a bin edge
+        assert np.abs(got - h).sum() <= 4
+    # spin correlations: shells 1 (A-B) and 2 (A-A, B-B), kinds 1, 2
+    sc_rows = [l.split() for l in (tmp_path / "bcc2_spincorr.dat").read_text().splitlines()
+               if l.strip() and not l.strip().startswith("#")]
+    assert len(sc_rows) == R * 2 * 2 * 2
+    assert {r[2] for r in sc_rows} == {"Fe", "Co"}
+    sc = np.array([[float(r[k]) for k in (0, 1, 3, 5, 6, 7, 8)] for r in sc_rows])
+    sc = np.insert(np.insert(sc, 2, 0.0, axis=1), 4, 0.0, axis=1)   # columns as in the file
+    kinds = np.arange(N) % 2
+    c1, n1 = Lt.shell_tables("bcc", 4, 2, builder=O.build_table_fn)
+    for t in range(R):
+        tmp, tab = B.spincorr(spins[t], kinds, c1, n1, np.ones(N), 2)
+        rows = sc[t * 8:(t + 1) * 8]
+        for row in rows:
+            l, a, b = int(row[1]) - 1, int(row[3]) - 1, int(row[5]) - 1
+            assert row[6] == pytest.approx(tmp[l, a, b] / N, abs=5e-5)
+            assert row[7] == pytest.approx(tab[l, a, b] / N, abs=5e-5)
+            assert row[8] == pytest.approx(abs(tmp[l, a, b]) / N, abs=5e-5)
+    # structure factor: 25 q-points per temperature, 5 columns with the polarisation
+    pos = Lt.site_positions("bcc", 4) * 2.0
+    tp = 2 * np.pi / 2.0
+    qs = [(lx / 4 * tp, 0.0, ly / 4 * tp) for lx in range(5) for ly in range(5)]
+    for t in range(R):
+        d = np.loadtxt(tmp_path / f"bcc2_sf-T{t + 1:03d}.dat")
+        assert d.shape == (25, 5)
+        want = B.sfactor(spins[t], pos, qs, polar=(0, 0, tp)) / N
+        assert np.allclose(d[:, 2:].T, want, rtol=1e-3, atol=2e-4 * N)
+
+
+def test_binning_error_file(tmp_path):
+    """binerror of E against the oracle's binning of the _energy-T series (E/N per measured step,
+    E16.8) -- the same series the engine's trace holds."""
+    from oracle import obs_oracle as B
+    extra = """
+ steps_warmup = 50
+ steps_mc = 64
+ steps_measure = 1
+ binning_error = .true.
+ binning_level = 1-3
+ energy_write = .true.
+"""
+    _two_kind_bcc(tmp_path, 5, extra)
+    _run(tmp_path, "bcc2")
+    lines = (tmp_path / "bcc2_binerror.dat").read_text().splitlines()
+    blocks = [i for i, l in enumerate(lines) if l.strip().startswith("#T=")]
+    assert len(blocks) == 2
+    for t, b in enumerate(blocks):
+        got = np.array([[float(v) for v in l.split()] for l in lines[b + 1:b + 4]])
+        assert got[:, 0].tolist() == [1, 2, 3]
+        e = np.loadtxt(tmp_path / f"bcc2_energy-T{t + 1:03d}.dat", skiprows=3)[:, 1]
+        assert len(e) == 64
+        var = B.binning(e[:, None], [1, 2, 3])
+        assert np.allclose(got[:, 1], np.sqrt(var[0] / 128), rtol=1e-4, atol=1e-12)
