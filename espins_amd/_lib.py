@@ -80,6 +80,8 @@ _SIGS = {
     "esp_sync": (_i, [_p]),
     "esp_stream": (_p, [_p]),
     "esp_copy_probe": (_i, [_p, C.c_int64, _i, _dp]),
+    "esp_neighbor_shells": (_i, [_i, _dp, _dp, _ip, _d, _i, _i, _dp, _ip, _ip]),
+    "esp_spin_glass": (_i, [_i, _i, _ip, _ip, _dp, _dp, _i, _ip]),
     "esp_set_orderparam": (_i, [_p, _dp]),
     "esp_set_staggered": (_i, [_p, _i, _dp]),
     "esp_set_pm": (_i, [_p, _i, _ip]),

@@ -41,7 +41,7 @@ def build_hip(verbose: bool = False) -> str:
     objdir = os.path.join(PKG, "_obj")
     os.makedirs(objdir, exist_ok=True)
     jobs = []
-    for src in ("esp_model.cpp", "esp_hip.hip", "esp_batch.hip", "esp_learn.hip", "esp_obs.hip"):
+    for src in ("esp_model.cpp", "esp_setup.cpp", "esp_hip.hip", "esp_batch.hip", "esp_learn.hip", "esp_obs.hip"):
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue

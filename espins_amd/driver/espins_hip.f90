@@ -17,7 +17,7 @@ module espins_hip
             esp_reset_stats, esp_sync, esp_error_text, esp_sweep, esp_learn_set_weights, &
             esp_learn_collect, esp_learn_train, esp_learn_clear, esp_set_trace, esp_get_trace, &
             esp_set_orderparam, esp_set_staggered, esp_set_pm, esp_set_spincorr, esp_set_sfactor, &
-            esp_get_obs, esp_get_obs_trace
+            esp_get_obs, esp_get_obs_trace, esp_spin_glass
 
   integer(c_int), parameter, public :: ESP_OBS_ORDER = 1, ESP_OBS_STAGGERED = 2, ESP_OBS_PM = 4, &
                                        ESP_OBS_SPINCORR = 8, ESP_OBS_SFACTOR = 16
@@ -138,6 +138,14 @@ module espins_hip
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: trace(*)
       integer(c_int), intent(out) :: n
+    end function
+    integer(c_int) function esp_spin_glass(N, z, cnt, nbr, val, sigma, seed, bad) bind(C, name='esp_spin_glass')
+      import :: c_int, c_double
+      integer(c_int), value :: N, z, seed
+      integer(c_int), intent(in) :: cnt(*), nbr(*)
+      real(c_double), intent(inout) :: val(*)
+      real(c_double), intent(in) :: sigma(*)
+      integer(c_int), intent(out) :: bad(2)
     end function
     ! optional observables
     integer(c_int) function esp_set_orderparam(ctx, axes) bind(C, name='esp_set_orderparam')

@@ -67,6 +67,7 @@ program mc_gpu
     mag(i) = inp%moment(mod(i - 1, inp%nb) + 1)
   end do
   call build(inp%J, 1, zJ, jc, jn, jv)
+  if (inp%spin_glass) call disorder()
   zB = 0
   zD = 0
   if (inp%have_B) call build(inp%B, 1, zB, bc, bn, bv)
@@ -272,6 +273,42 @@ program mc_gpu
 224 format(3x,f9.4,1x,i14,1x,i14,5x,i14,7x,f10.8,4x,a)   ! pt.F90:1126
 
 contains
+
+  ! Spin glass (jij_spin_glass, jij.F90:506-556): the sigma of every slot of the J table from
+  ! the sig= of the row that made it (a table of row numbers built over the same rows, zero-J
+  ! rows dropped as in the J table), sigma / k_B as jij.F90:164 leaves it, then Gaussian
+  ! disorder from the MT19937 stream of spin_glass_seed (/dev/urandom without it).
+  subroutine disorder()
+    integer(c_int), allocatable :: ic(:), in(:)
+    real(c_double), allocatable :: iv(:), ids(:), sg(:), ones(:)
+    integer(c_int) :: z, st, bad(2), sgs
+    integer :: k
+    allocate(ids(inp%J%n), ones(inp%nb))
+    ones = 1
+    do k = 1, inp%J%n
+      ids(k) = merge(real(k, dp), 0.0_dp, inp%J%val(k) /= 0)
+    end do
+    allocate(ic(N), in(N * max(inp%J%n, 1)), iv(N * max(inp%J%n, 1)))
+    st = esp_build_table(int(inp%nb, c_int), inp%basis, ones, inp%L, merge(1_c_int, 0_c_int, inp%open_boundary), &
+                         int(inp%J%n, c_int), inp%J%f1, inp%J%f2, ids, c_null_ptr, 1_c_int, &
+                         int(max(inp%J%n, 1), c_int), ic, in, iv, z)
+    if (st /= ESP_OK .or. z /= zJ) call input_error('spin glass: row table does not match the J table')
+    allocate(sg(N * max(zJ, 1)))
+    sg = 0
+    do k = 1, N * zJ
+      if (nint(iv(k)) > 0) sg(k) = inp%J%sigma(nint(iv(k))) / k_B
+    end do
+    sgs = inp%sg_seed
+    if (.not. inp%have_sg_seed) sgs = urandom_seed()
+    write(out_u, '(1x,a,i12)') 'Spin glass seed            : ', sgs
+    st = esp_spin_glass(int(N, c_int), zJ, jc, jn, jv, sg, sgs, bad)
+    if (st /= ESP_OK) then
+      write(out_u, '(2x,a)') 'Jij_parameters block is not symmetric.'
+      write(out_u, '(2x,a,i4,a,i4,a,5x,a,i4,a,i4,a)') 'Asymmetric pairs are (', bad(1), ',', bad(2), ')', &
+                                                     '(', bad(2), ',', bad(1), ')'
+      call input_error('Found asymmetric pairs in Jij_parameters block')
+    end if
+  end subroutine
 
   ! Optional observables on the device (esp_set_orderparam / _staggered / _pm / _spincorr /
   ! _sfactor) from the input's flags; P(m) is always on, as the reference always writes _pm.dat.

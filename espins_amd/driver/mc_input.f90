@@ -20,6 +20,7 @@ module mc_input
     integer :: n = 0
     real(dp), allocatable :: f1(:,:), f2(:,:), val(:), dvec(:,:)
     integer, allocatable :: shell(:)            ! sh= of each row (0 when absent)
+    real(dp), allocatable :: sigma(:)           ! sig= of each row (eV, 0 when absent)
   end type
 
   type, public :: input_t
@@ -55,6 +56,9 @@ module mc_input
     integer :: sf_mesh(2) = 50, sf_steps_measure = 100
     real(dp) :: sf_corner(3) = 0, sf_q1(3) = 0, sf_q2(3) = 0, sf_polar(3) = 0
     real(dp) :: recip(3,3) = 0                  ! recip(:,k) = reciprocal vector k (2 pi / Angstrom)
+    ! spin-glass disorder of the exchange (parameters.F90:616-630, jij.F90:506-556)
+    logical :: spin_glass = .false., have_sg_seed = .false.
+    integer :: sg_seed = 0
   end type
 
   public :: read_input, input_error
@@ -273,8 +277,9 @@ contains
       if (len_trim(lines(k)) > 0) n = n + 1
     end do
     r%n = n
-    allocate(r%f1(3,n), r%f2(3,n), r%val(n), r%shell(n))
+    allocate(r%f1(3,n), r%f2(3,n), r%val(n), r%shell(n), r%sigma(n))
     r%shell = 0
+    r%sigma = 0
     n = 0
     do k = first, last
       if (len_trim(lines(k)) == 0) cycle
@@ -311,6 +316,10 @@ contains
         if (tok(p)(1:3) == 'sh=') then
           read(tok(p)(4:), *, iostat=ios) r%shell(n)
           if (ios /= 0) call input_error('bad sh= in '//trim(lines(k)))
+        else if (tok(p)(1:4) == 'sig=') then
+          read(tok(p)(5:), *, iostat=ios) r%sigma(n)
+          if (ios /= 0) call input_error('bad sig= in '//trim(lines(k)))
+          r%sigma(n) = r%sigma(n) * econv
         end if
       end do
     end do
@@ -720,10 +729,15 @@ contains
     subroutine unsupported_keywords()
       character(len=maxl) :: w
       call keyword('spin_glass', found, w)
-      if (found) then
-        if (is_true(w)) call input_error('spin_glass = true is not supported by mc_gpu.x')
-      end if
+      if (found) inp%spin_glass = is_true(w)
       call keyword('spin_glass_seed', found, w)
+      if (found) then
+        read(w, *, iostat=ios) inp%sg_seed
+        if (ios /= 0) call input_error('bad spin_glass_seed')
+        inp%have_sg_seed = .true.
+      end if
+      if (inp%spin_glass .and. any(inp%J%sigma < 0)) &
+        call input_error('sig in jij_parameters block must be positive')
     end subroutine
 
     ! Keywords of the reference that only its input-generation modes (-inp1/-inp2/-ham) read,

@@ -167,7 +167,7 @@ def parse(text: str) -> McinInput:
                 elif u not in ("ev", "ang"):
                     raise McinError(f"Units in block {name} not recognised")
             body = body[1:]
-        f1, f2, val = [], [], []
+        f1, f2, val, sh, sig = [], [], [], [], []
         for r in body:
             parts = [p.strip() for p in r.replace(" ", "").split(":")]
             def site(p, tag):
@@ -182,7 +182,18 @@ def parse(text: str) -> McinInput:
             if not parts[2].startswith(key + "="):
                 raise McinError(f"malformed parameters definition: {r}")
             val.append(float(parts[2][len(key) + 1:]))
-        return Rows(np.array(f1).reshape(-1, 3), np.array(f2).reshape(-1, 3), np.array(val) * econv)
+            s_, g_ = 0, 0.0                 # optional sh= and sig= (parameters.F90:2751-2766)
+            for p in parts[3:]:
+                if p.startswith("sh="):
+                    s_ = int(p[3:])
+                elif p.startswith("sig="):
+                    g_ = float(p[4:])
+            sh.append(s_)
+            sig.append(g_)
+        out = Rows(np.array(f1).reshape(-1, 3), np.array(f2).reshape(-1, 3), np.array(val) * econv)
+        out.shell = np.array(sh, np.int32)
+        out.sigma = np.array(sig) * econv
+        return out
 
     rows = {}
     rj = rows_of("jij_parameters", "jij")
@@ -251,6 +262,27 @@ def read(path: str) -> McinInput:
         return parse(f.read())
 
 
+def _spin_glass(inp, L, ob, J, builder):
+    """jij_spin_glass (jij.F90:506-556) on the J table: the sigma of each slot is the sig= of the
+    row that made it (a table of row numbers over the same rows, zero-J rows dropped like the J
+    table), over k_B (jij.F90:164); the disorder comes from spin_glass_seed."""
+    from .setup_gen import spin_glass
+    rows = inp.rows["J"]
+    seed = inp.kw("spin_glass_seed", None, int)
+    if seed is None:
+        raise McinError("spin_glass needs spin_glass_seed here (the stream must be reproducible)")
+    if (rows.sigma < 0).any():
+        raise McinError("sig in jij_parameters block must be positive")
+    ids = np.where(rows.value != 0.0, np.arange(1, rows.n + 1), 0).astype(float)
+    nb = len(inp.labels)
+    t = build_table(nb, inp.basis_frac, np.ones(nb), L, Rows(rows.f1, rows.f2, ids), 1, ob, builder)
+    if t.z != J.z:
+        raise McinError("spin glass: row table does not match the J table")
+    idx = np.rint(t.val).astype(int)
+    sigma = np.where(idx > 0, rows.sigma[np.maximum(idx, 1) - 1] / K_B, 0.0)
+    return spin_glass(J, sigma, seed)
+
+
 def to_model(inp: McinInput, supercell=None, builder=None, open_boundary=None) -> Model:
     """Build the reference's tables (internal units) for this input."""
     L = tuple(supercell or inp.supercell)
@@ -265,6 +297,8 @@ def to_model(inp: McinInput, supercell=None, builder=None, open_boundary=None) -
     J = build_table(nb, inp.basis_frac, inp.moments, L, conv(inp.rows["J"]), 1, ob, builder) \
         if "J" in inp.rows else Table(0, np.zeros(N, np.int32), np.zeros((0, N), np.int32),
                                       np.zeros((0, N)))
+    if "J" in inp.rows and inp.kw("spin_glass", ".false.").lstrip(".").startswith("t"):
+        J = _spin_glass(inp, L, ob, J, builder)
     m = Model("mcin", L, nb, inp.basis_frac, mag, J, rows=dict(inp.rows))
     if "B" in inp.rows:
         m.B = build_table(nb, inp.basis_frac, inp.moments, L, conv(inp.rows["B"]), 1, ob, builder)

@@ -63,6 +63,20 @@ int esp_build_table(int nb, const double *basis_frac, const double *basis_mag,
                     const double *dvec, int ncomp, int zcap,
                     int *cnt, int *nbr, double *val, int *zmax);
 
+/* neighbors_get (neighbors.F90:48-168) for the input generation: for atom a of the cell
+ * (Cartesian pos[nb][3], lattice vectors lat[3][3] as rows), its first nshell[a] neighbour
+ * shells among the images l a1 + m a2 + n a3, |l|,|m|,|n| <= 5, distances equal within tol:
+ * dnn[nb][cap_shell] shell radii, multi[nb][cap_shell] multiplicities, list[nb][cap_shell]
+ * [cap_multi][4] = (atom b, l, m, n) in the reference's order.  Host only. */
+int esp_neighbor_shells(int nb, const double *lat, const double *pos, const int *nshell, double tol,
+                        int cap_shell, int cap_multi, double *dnn, int *multi, int *list);
+/* jij_spin_glass (jij.F90:506-556): J += sigma x N(0,1) (MT19937 stream of seed, polar
+ * method) for every bond i < j of the ELL table in table order, copied to the first (j, i)
+ * slot.  sigma[z][N] as val.  ESP_ETABLE if a bond has no reverse (bad_pair = i, j).
+ * Host only. */
+int esp_spin_glass(int N, int z, const int *cnt, const int *nbr, double *val, const double *sigma, int seed,
+                   int *bad_pair);
+
 /* ---- model ------------------------------------------------------------------------ */
 /* Upload the Hamiltonian (the tables energy_get / delta_energy_get read,
  * get_quant.F90:186-337).  zB = 0 / zD = 0 / have_si = 0 / have_field = 0 switch a term

@@ -50,5 +50,8 @@ def test_observable_keyword_checks(tmp_path):
     assert r.returncode != 0 and "sfactor_2dqmesh must be greater than zero" in r.stdout
     r = _run(tmp_path, _obs_input(sfactor_q1="1 0 0", sfactor_q2="2 0 0"))
     assert r.returncode != 0 and "not linearly independent" in r.stdout
-    r = _run(tmp_path, _obs_input(spin_glass=".true."))
-    assert r.returncode != 0 and "spin_glass = true is not supported" in r.stdout
+    lines = _obs_input(spin_glass=".true.").splitlines()
+    k = next(i for i, l in enumerate(lines) if "jij=" in l.replace(" ", "").lower())
+    lines[k] = lines[k].split("!")[0].rstrip() + ":sig=-0.001"
+    r = _run(tmp_path, "\n".join(lines) + "\n")
+    assert r.returncode != 0 and "sig in jij_parameters block must be positive" in r.stdout

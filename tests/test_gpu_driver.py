@@ -330,3 +330,35 @@ def test_binning_error_file(tmp_path):
         assert len(e) == 64
         var = B.binning(e[:, None], [1, 2, 3])
         assert np.allclose(got[:, 1], np.sqrt(var[0] / 128), rtol=1e-4, atol=1e-12)
+
+
+def test_spin_glass_initial_energy(tmp_path):
+    """spin_glass = true with sig= on every J row: the driver's initial (ferro) energy equals the
+    oracle's energy of the same state on the disordered table mcin.to_model builds (same rows,
+    same spin_glass_seed, esp_spin_glass on both sides), and differs from the clean one."""
+    from espins_amd import lattice as Lt
+    from espins_amd import mcin
+    from oracle import oracle as O
+    text = open(os.path.join(GOLDEN, "examples", "ex01_cubic.mcin")).read()
+    out = []
+    for l in text.splitlines():
+        if "jij=" in l.replace(" ", "").lower() and l.strip().lower().startswith(("f1", "c1")):
+            l = l.split("!")[0].rstrip() + ":sig=0.003"
+        out.append(l)
+    text = "\n".join(out) + "\n spin_glass = .true.\n spin_glass_seed = 11\n"
+    (tmp_path / "sg.mcin").write_text(text)
+    p = tmp_path / "sg.mcin"
+    t = p.read_text()
+    for key, val in {"steps_warmup": 10, "steps_mc": 10, "seeds": 3}.items():
+        pat = re.compile(rf"^\s*!?!?\s*{key}\s*[=:].*$", re.I | re.M)
+        t = pat.sub(f" {key} = {val}", t, count=1) if pat.search(t) else f" {key} = {val}\n" + t
+    p.write_text(t)
+    res = _run(tmp_path, "sg")
+    e0 = [float(x) for x in re.findall(r"Initial Energy\s*=\s*(\S+)", res)]
+    assert "Spin glass seed" in res and "11" in res
+    inp = mcin.read(str(p))
+    m = mcin.to_model(inp, builder=O.build_table_fn)
+    want = O.Oracle(m).energy(Lt.ferro_spins(m.N, 1, m.mag)[0])
+    assert len(e0) == 20
+    assert np.allclose(e0, want, rtol=2e-7)
+    assert abs(want - (-0.6684205e4)) > 1.0
