@@ -49,7 +49,11 @@ constexpr int kLdsBlock = 512;   // LDS-resident kernel (8 waves per replica)
 #endif
 constexpr int kChunk = ESP_CHUNK;  // partner gathers issued together per lane
 constexpr int kChunkWide = ESP_CHUNK_WIDE;  // ... for launches too small to fill the GPU with waves
-constexpr int kChunkLds = 8;     // ... in the LDS-resident sweep and the learned sweep
+constexpr int kChunkLds = 8;     // ... in the LDS-resident sweep
+#ifndef ESP_CHUNK_LEARN
+#define ESP_CHUNK_LEARN 8
+#endif
+constexpr int kChunkLearn = ESP_CHUNK_LEARN;  // ... in the learned sweep
 // Rejections are counted per slot in kRejSub sub-counters, rej[slot][kRejSub]: a wave adds
 // to the sub-counter its global wave index selects, so the thousands of waves of one replica
 // do not queue at one address (a single counter per slot cost 12-14 % of a colour phase).

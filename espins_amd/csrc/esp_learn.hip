@@ -326,18 +326,18 @@ __device__ __forceinline__ void local_env(const KArgs &a, const SP &sp, const do
   for (int k = 0; k < 6; ++k) QB[k] = 0.0;
   const double *crow = CMP ? ctab + (size_t)a.basis[d] * a.zmax * 5 : nullptr;
   const unsigned ud = (unsigned)d;
-  for (int l0 = 0; l0 < a.zmax; l0 += kChunkLds) {
-    unsigned jj[kChunkLds];
-    V3 sv[kChunkLds];
+  for (int l0 = 0; l0 < a.zmax; l0 += kChunkLearn) {
+    unsigned jj[kChunkLearn];
+    V3 sv[kChunkLearn];
 #pragma unroll
-    for (int u = 0; u < kChunkLds; ++u) {
+    for (int u = 0; u < kChunkLearn; ++u) {
       const int l = l0 + u < a.zmax ? l0 + u : 0;
       jj[u] = (unsigned)ld_off(a.nbr, ((unsigned)l * (unsigned)a.N + ud) * 4u);
     }
 #pragma unroll
-    for (int u = 0; u < kChunkLds; ++u) sv[u] = sp.ld(jj[u]);
+    for (int u = 0; u < kChunkLearn; ++u) sv[u] = sp.ld(jj[u]);
 #pragma unroll
-    for (int u = 0; u < kChunkLds; ++u) {
+    for (int u = 0; u < kChunkLearn; ++u) {
       const int l = l0 + u;
       if (l >= a.zmax) continue;
       const V3 sj = sv[u];

@@ -362,3 +362,25 @@ def test_spin_glass_initial_energy(tmp_path):
     assert len(e0) == 20
     assert np.allclose(e0, want, rtol=2e-7)
     assert abs(want - (-0.6684205e4)) > 1.0
+
+
+def test_pt_learned_proposal(tmp_path):
+    """Parallel tempering with mcarlo_mode = learned: training after warmup2, the learned sweep
+    captured in the PT rounds; the final replica map is a permutation and the running energies
+    equal the direct ones at the end (ENERGY3 rows)."""
+    _prepare(tmp_path, "ex02_cubic.mcin", "cubic",
+             {"steps_warmup": 400, "steps_mc": 600, "supercell_size": "6 6 6", "seeds": 21,
+              "energy_write": ".false.", "mcarlo_mode": "learned", "learned_records": 32768,
+              "learned_epochs": 2})
+    out = _run(tmp_path, "cubic")
+    assert "Learned proposal: training on" in out
+    tail = out[out.rindex("Final configuration =") + len("Final configuration ="):].split()
+    perm = [int(v) for v in tail[:32]]           # list-directed output wraps over lines
+    assert sorted(perm) == list(range(1, 33))
+    rows = [l.split() for l in out.splitlines() if "<-- ENERGY3" in l]
+    adds = [float(r[-3]) for r in rows if "adding" in " ".join(r)]
+    direct = [float(r[-3]) for r in rows if "direct" in " ".join(r)]
+    assert len(adds) == len(direct) == 32
+    assert np.allclose(adds, direct, rtol=1e-6)
+    mine = np.loadtxt(tmp_path / "cubic_mc.dat", comments="#")
+    assert mine.shape == (32, 7) and np.isfinite(mine).all()
