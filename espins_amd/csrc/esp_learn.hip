@@ -10,7 +10,7 @@
 // neighbours, which a colour phase never changes, so detailed balance holds for any weights:
 // the network only decides how often moves are accepted.
 //
-// Network: 5 -> 32 -> 32 -> 32 -> 6, GELU, BF16 weights for sampling (FP32 accumulate), FP32
+// Network: 5 -> 32 -> 32 -> 32 -> 6, hard-swish, BF16 weights for sampling (FP32 accumulate), FP32
 // master weights for training with Adam.  Outputs per component k: log-scale s_k of the
 // concentration kappa_k = beta|G| exp(s_k), weight logit, tilt t_k = (pi/2) tanh(.) of the
 // mean direction from G-hat toward the nearer pole of the anisotropy axis.
@@ -42,20 +42,14 @@ constexpr double kPi = 3.14159265358979323846;
 // at most a factor kUniform against uniform proposals instead of freezing it.
 constexpr double kUniform = 1.0 / 32.0;
 
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
-// GELU with erf from Abramowitz & Stegun 7.1.26 (|error| <= 1.5e-7): branch-free, one exp2
-// and one reciprocal.  The activation of the sweep's network.
-__device__ __forceinline__ float gelu_fast(float x) {
-  const float y = fabsf(x) * 0.70710678118654752f;
-  const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, y, 1.0f));
-  const float p =
-      t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 1.061405429f, -1.453152027f), 1.421413741f), -0.284496736f), 0.254829592f);
-  const float e = p * __builtin_amdgcn_exp2f(-y * y * 1.4426950408889634f);   // 1 - erf(|x| / sqrt 2)
-  return 0.5f * x * (x >= 0.0f ? 2.0f - e : e);
-}
-
-__device__ __forceinline__ float gelu_d(float x) {
-  return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
+// Activation: hard-swish, h(x) = x clamp(x / 6 + 1/2, 0, 1) (Howard et al. 2019, "Searching
+// for MobileNetV3").  Two or three VALU instructions and no transcendental, so the sweep's 96
+// activations per site cost little beside the MFMA layers; no approximation is involved, so
+// training and sweep evaluate the same FP32 expression.  h'(x) = 0 for x <= -3, 1 for x >= 3, x / 3 + 1/2
+// between.
+__device__ __forceinline__ float act(float x) { return x * fminf(fmaxf(fmaf(x, 1.0f / 6.0f, 0.5f), 0.0f), 1.0f); }
+__device__ __forceinline__ float act_d(float x) {
+  return x <= -3.0f ? 0.0f : (x >= 3.0f ? 1.0f : fmaf(x, 1.0f / 3.0f, 0.5f));
 }
 
 // MLP forward (weights w in any address space).
@@ -66,21 +60,21 @@ __device__ __forceinline__ void mlp_forward(const float *w, const float x[kIn], 
     float s = w[oB1 + i];
 #pragma unroll
     for (int k = 0; k < kIn; ++k) s += w[oW1 + i * kIn + k] * x[k];
-    h1[i] = gelu(s);
+    h1[i] = act(s);
   }
 #pragma unroll 2
   for (int i = 0; i < kH; ++i) {
     float s = w[oB2 + i];
 #pragma unroll 8
     for (int k = 0; k < kH; ++k) s += w[oW2 + i * kH + k] * h1[k];
-    h2[i] = gelu(s);
+    h2[i] = act(s);
   }
 #pragma unroll 2
   for (int i = 0; i < kH; ++i) {
     float s = w[oB3 + i];
 #pragma unroll 8
     for (int k = 0; k < kH; ++k) s += w[oW3 + i * kH + k] * h2[k];
-    h1[i] = gelu(s);  // h3
+    h1[i] = act(s);  // h3
   }
 #pragma unroll
   for (int i = 0; i < kOut; ++i) {
@@ -115,11 +109,12 @@ constexpr double kLog2Pi = 1.8378770664093453;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {   // round to nearest even
-  const uint32_t a = __float_as_uint(lo), b = __float_as_uint(hi);
-  const uint32_t ra = (a + 0x7FFFu + ((a >> 16) & 1u)) >> 16;
-  const uint32_t rb = (b + 0x7FFFu + ((b >> 16) & 1u)) >> 16;
-  return ra | (rb << 16);
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+// Two floats to BF16, round to nearest even, lo in bits 0-15: one v_cvt_pk_bf16_f32 on gfx950.
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+  const floatx2 v = {lo, hi};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
 
 __device__ __forceinline__ bf16x8 frag(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3) {
@@ -192,7 +187,7 @@ __device__ __forceinline__ void net_forward_mfma(const float *wl, const NetA &na
           float acc = wl[oB1 + u];
 #pragma unroll
           for (int k = 0; k < kIn; ++k) acc += wl[oW1 + u * kIn + k] * x[k];
-          z[t] = gelu_fast(acc);
+          z[t] = act(acc);
         }
         (half ? snd : own)[4 * ks + e / 2] = pack_bf16(z[0], z[1]);
       }
@@ -218,7 +213,7 @@ __device__ __forceinline__ void net_forward_mfma(const float *wl, const NetA &na
       z[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(na.a[layer][0], b[cb][0], z[cb], 0, 0, 0);
       z[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(na.a[layer][1], b[cb][1], z[cb], 0, 0, 0);
 #pragma unroll
-      for (int v = 0; v < 16; ++v) z[cb][v] = gelu_fast(z[cb][v]);
+      for (int v = 0; v < 16; ++v) z[cb][v] = act(z[cb][v]);
     }
     if (layer == 0)
 #pragma unroll
@@ -417,8 +412,17 @@ __device__ __forceinline__ void axis_of(const KArgs &a, int d, V3 &ax, double &D
 }
 
 // One colour phase with learned proposals (global-memory path, one lane per site).
+#ifndef ESP_LEARN_WAVES
+#define ESP_LEARN_WAVES 4        // > 0: ask the compiler for this many waves per SIMD (4: no spills, no AGPRs)
+#endif
+#if ESP_LEARN_WAVES > 0
+#define ESP_LEARN_ATTR __attribute__((amdgpu_waves_per_eu(ESP_LEARN_WAVES)))
+#else
+#define ESP_LEARN_ATTR
+#endif
+
 template <bool CMP, bool HB, bool HD>
-__global__ __launch_bounds__(kBlock) void k_phase_learned(KArgs a, const uint16_t *wbf16) {
+__global__ __launch_bounds__(kBlock) ESP_LEARN_ATTR void k_phase_learned(KArgs a, const uint16_t *wbf16) {
   extern __shared__ double s_dyn[];  // [ctab][weights as float]
   int r, b;
   decode_block(a, r, b);
@@ -553,19 +557,19 @@ __global__ __launch_bounds__(kTrainBlock) void k_train_block(const float *w, con
       float z = w[oB1 + q];
       for (int k = 0; k < kIn; ++k) z += w[oW1 + q * kIn + k] * a0[k];
       d1[q] = z;
-      a1[q] = gelu(z);
+      a1[q] = act(z);
     }
     for (int q = 0; q < kH; ++q) {
       float z = w[oB2 + q];
       for (int k = 0; k < kH; ++k) z += w[oW2 + q * kH + k] * a1[k];
       d2[q] = z;
-      a2[q] = gelu(z);
+      a2[q] = act(z);
     }
     for (int q = 0; q < kH; ++q) {
       float z = w[oB3 + q];
       for (int k = 0; k < kH; ++k) z += w[oW3 + q * kH + k] * a2[k];
       d3[q] = z;
-      a3[q] = gelu(z);
+      a3[q] = act(z);
     }
     float o[kOut];
     for (int q = 0; q < kOut; ++q) {
@@ -600,17 +604,17 @@ __global__ __launch_bounds__(kTrainBlock) void k_train_block(const float *w, con
     for (int k = 0; k < kH; ++k) {
       float acc = 0.f;
       for (int q = 0; q < kOut; ++q) acc += w[oW4 + q * kH + k] * g4[q];
-      d3[k] = acc * gelu_d(d3[k]);
+      d3[k] = acc * act_d(d3[k]);
     }
     for (int k = 0; k < kH; ++k) {
       float acc = 0.f;
       for (int q = 0; q < kH; ++q) acc += w[oW3 + q * kH + k] * d3[q];
-      d2[k] = acc * gelu_d(d2[k]);
+      d2[k] = acc * act_d(d2[k]);
     }
     for (int k = 0; k < kH; ++k) {
       float acc = 0.f;
       for (int q = 0; q < kH; ++q) acc += w[oW2 + q * kH + k] * d2[q];
-      d1[k] = acc * gelu_d(d1[k]);
+      d1[k] = acc * act_d(d1[k]);
     }
   } else {
     for (int k = 0; k < kS; ++k) my[k] = 0.f;

@@ -23,7 +23,6 @@ from __future__ import annotations
 import math
 
 import numpy as np
-from scipy.special import erf
 
 NIN, NH, NOUT = 5, 32, 6
 UNIFORM = 1.0 / 32.0          # weight of the fixed uniform-sphere component
@@ -47,23 +46,25 @@ def pack(p):
     return np.concatenate([np.asarray(p[name], np.float64).ravel() for name, _ in _SHAPES])
 
 
-def gelu(x):
-    return 0.5 * x * (1.0 + erf(x / math.sqrt(2.0)))
+def act(x):
+    """Hard-swish, x clip(x / 6 + 1/2, 0, 1) (Howard et al. 2019, MobileNetV3)."""
+    return x * np.clip(x / 6.0 + 0.5, 0.0, 1.0)
 
 
-def gelu_d(x):
-    return 0.5 * (1.0 + erf(x / math.sqrt(2.0))) + x * np.exp(-0.5 * x * x) / math.sqrt(2.0 * math.pi)
+def act_d(x):
+    """Its derivative, 0 at and below -3 and 1 at and above 3."""
+    return np.where(x <= -3.0, 0.0, np.where(x >= 3.0, 1.0, x / 3.0 + 0.5))
 
 
 def forward(w, X):
     """Outputs [n][6] and the pre-activations for the backward pass."""
     p = unpack(w)
     z1 = X @ p["W1"].T + p["b1"]
-    a1 = gelu(z1)
+    a1 = act(z1)
     z2 = a1 @ p["W2"].T + p["b2"]
-    a2 = gelu(z2)
+    a2 = act(z2)
     z3 = a2 @ p["W3"].T + p["b3"]
-    a3 = gelu(z3)
+    a3 = act(z3)
     o = a3 @ p["W4"].T + p["b4"]
     return o, (X, z1, a1, z2, a2, z3, a3)
 
@@ -79,9 +80,9 @@ def forward_sweep_net(w, X):
     """The network as the sweep evaluates it: BF16 weights; the inputs of the two 32 x 32
     layers rounded to BF16; sums and activations otherwise in full precision."""
     p = unpack(bf16_round(w))
-    a1 = bf16_round(gelu(X @ p["W1"].T + p["b1"]))
-    a2 = bf16_round(gelu(a1 @ p["W2"].T + p["b2"]))
-    a3 = gelu(a2 @ p["W3"].T + p["b3"])
+    a1 = bf16_round(act(X @ p["W1"].T + p["b1"]))
+    a2 = bf16_round(act(a1 @ p["W2"].T + p["b2"]))
+    a3 = act(a2 @ p["W3"].T + p["b3"])
     return a3 @ p["W4"].T + p["b4"]
 
 
@@ -173,11 +174,11 @@ def loss_and_grad(w, recs):
     g /= n
     p = unpack(w)
     gr = {"W4": g.T @ a3, "b4": g.sum(0)}
-    d3 = (g @ p["W4"]) * gelu_d(z3)
+    d3 = (g @ p["W4"]) * act_d(z3)
     gr["W3"], gr["b3"] = d3.T @ a2, d3.sum(0)
-    d2 = (d3 @ p["W3"]) * gelu_d(z2)
+    d2 = (d3 @ p["W3"]) * act_d(z2)
     gr["W2"], gr["b2"] = d2.T @ a1, d2.sum(0)
-    d1 = (d2 @ p["W2"]) * gelu_d(z1)
+    d1 = (d2 @ p["W2"]) * act_d(z1)
     gr["W1"], gr["b1"] = d1.T @ X, d1.sum(0)
     return float(-np.mean(lse)), pack(gr)
 

@@ -267,12 +267,11 @@ def test_bf16_round_matches_torch_bfloat16():
 def test_forward_sweep_net_hand_computed_case():
     """One path through the network with hand-chosen weights: x0 = 1 feeds hidden unit 0 of
     layer 1 with weight 1; layers 2 and 3 copy unit 0 with weight 1; output 0 reads unit 0 with
-    weight 1 and output 1 is the bias 0.1.
-      a1 = bf16(gelu(1))            = bf16(0.84134474...) = 0.83984375 (= 215/256)
-      a2 = bf16(gelu(0.83984375))   = bf16(0.67080...)    = 0.671875   (= 172/256)
-      a3 = gelu(0.671875)           (full precision)
-      o0 = a3, o1 = bf16(0.1) = 0.10009765625, others 0
-    gelu(x) = x Phi(x) (exact erf)."""
+    weight 1 and output 1 is the bias 0.1.  With h(x) = x clip(x / 6 + 1/2, 0, 1):
+      a1 = bf16(h(1))               = bf16(2/3)           = 0.66796875 (= 171/256)
+      a2 = bf16(h(0.66796875))      = bf16(0.40835571...) = 0.408203125 (= 209/512)
+      a3 = h(0.408203125)           (full precision)
+      o0 = a3, o1 = bf16(0.1) = 0.10009765625, others 0"""
     p = {k: np.zeros(s) for k, s in LO._SHAPES}
     p["W1"][0, 0] = 1.0
     p["W2"][0, 0] = 1.0
@@ -281,20 +280,18 @@ def test_forward_sweep_net_hand_computed_case():
     p["b4"][1] = 0.1
     w = LO.pack(p)
     X = np.array([[1.0, 0.0, 0.0, 0.0, 0.0]])
-    phi = lambda t: 0.5 * (1.0 + math.erf(t / math.sqrt(2.0)))
-    a1 = 215.0 / 256.0
-    assert abs(1.0 * phi(1.0) - a1) < 2.0 ** -9        # within half a BF16 ulp at this magnitude
-    a2 = 172.0 / 256.0
-    assert abs(a1 * phi(a1) - a2) < 2.0 ** -9
-    a3 = a2 * phi(a2)
+    h = lambda t: t * min(max(t / 6.0 + 0.5, 0.0), 1.0)
+    a1 = 171.0 / 256.0
+    assert abs(h(1.0) - a1) < 2.0 ** -9        # within half a BF16 ulp at this magnitude
+    a2 = 209.0 / 512.0
+    assert abs(h(a1) - a2) < 2.0 ** -10
+    a3 = h(a2)
     o = LO.forward_sweep_net(w, X)[0]
     assert o[0] == pytest.approx(a3, rel=1e-14)
     assert o[1] == 0.10009765625
     assert np.all(o[2:] == 0.0)
     # the full-precision network keeps the unrounded chain and differs from it
     of, _ = LO.forward(w, X)
-    want = (phi(1.0) * 1.0)
-    want = want * phi(want)
-    want = want * phi(want)
+    want = h(h(h(1.0)))
     assert of[0, 0] == pytest.approx(want, rel=1e-14)
     assert abs(of[0, 0] - o[0]) > 1e-4
