@@ -1,6 +1,7 @@
 ! mc_gpu.f90 -- driver of the MI355X engine with the reference's command line, input file and
 ! output layout:  mc_gpu.x seedname  reads seedname.mcin and writes seedname.mcout,
-! seedname_mc.dat and seedname_sconfig.dat.
+! seedname_mc.dat and seedname_sconfig.dat.  mc_gpu.x -dryrun seedname only reads the input
+! and writes the parameter echo.
 !
 ! Plain MC follows mcarlo (mcarlo.F90:463-859) with every temperature an independent chain on
 ! the device (the reference carries the configuration from one temperature to the next on a
@@ -11,6 +12,7 @@ program mc_gpu
   use iso_c_binding
   use espins_hip
   use mc_input
+  use mc_echo
   use mc_rng
   implicit none
 
@@ -37,12 +39,18 @@ program mc_gpu
   real(c_double), allocatable :: e_init(:), e_slot(:), d_slot(:)
   integer(c_int) :: nfreq
   integer :: nshell
+  integer :: sg_used = 0                 ! seed of the spin-glass normals
+  logical :: dryrun = .false.            ! -dryrun: read the input, write the echo, stop
 
   narg = command_argument_count()
   if (narg < 1) then
-    write(*, '(a)') 'usage: mc_gpu.x seedname'
+    write(*, '(a)') 'usage: mc_gpu.x [-dryrun] seedname'
     stop 1
   end if
+  do k = 1, narg - 1
+    call get_command_argument(k, arg)
+    if (trim(arg) == '-dryrun') dryrun = .true.
+  end do
   call get_command_argument(narg, arg)
   seedname = arg
   k = index(seedname, '.mcin', back=.true.)
@@ -51,13 +59,17 @@ program mc_gpu
 
   call read_input(trim(seedname)//'.mcin', inp)
   open(newunit=out_u, file=trim(seedname)//'.mcout', status='replace', action='write')
-  write(out_u, '(1x,a)') 'ESpinS-compatible Monte Carlo on MI355X (libespins_hip)'
-  write(out_u, '(1x,a,i6)') 'Library version            : ', esp_version()
-  write(out_u, '(1x,a,3i4)') 'Supercell size             : ', inp%L
-  write(out_u, '(1x,a,i6)') 'Atoms in the unit cell     : ', inp%nb
-  write(out_u, '(1x,a,l2)') 'Parallel tempering         : ', inp%pt
-  write(out_u, '(1x,a,a)')  'Monte Carlo mode           : ', trim(inp%mcarlo_mode)
-  write(out_u, '(1x,a,a)')  'Initial spins              : ', trim(inp%initial)
+  write(out_u, '(1x,a,i6,a)') 'ESpinS-compatible Monte Carlo on MI355X (libespins_hip version', esp_version(), ')'
+  write(out_u, '(/,1x,a)') 'Running on one GPU'
+  call write_echo(out_u, inp)
+  call system_clock(t1)
+  write(out_u, '(1x,a25,f11.3,a)') 'Time to read parameters  ', real(t1 - t0, dp) / real(rate, dp), ' (sec)'
+  t0 = t1
+  if (dryrun) then
+    write(out_u, '(/a)') ' Exiting... '//trim(seedname)//'.mcin read (-dryrun).'
+    close(out_u)
+    stop
+  end if
 
   ! ---- tables (jij_get / bij_get / dij_get through esp_build_table) -------------------
   ncell = inp%L(1) * inp%L(2) * inp%L(3)
@@ -70,8 +82,21 @@ program mc_gpu
   if (inp%spin_glass) call disorder()
   zB = 0
   zD = 0
-  if (inp%have_B) call build(inp%B, 1, zB, bc, bn, bv)
-  if (inp%have_D) call build(inp%D, 3, zD, dc, dn, dvv)
+  call system_clock(t1)
+  write(out_u, '(1x,a25,f11.3,a)') 'Time to get Jij matrix   ', real(t1 - t0, dp) / real(rate, dp), ' (sec)'
+  t0 = t1
+  if (inp%have_B) then
+    call build(inp%B, 1, zB, bc, bn, bv)
+    call system_clock(t1)
+    write(out_u, '(1x,a25,f11.3,a)') 'Time to get Bij matrix   ', real(t1 - t0, dp) / real(rate, dp), ' (sec)'
+    t0 = t1
+  end if
+  if (inp%have_D) then
+    call build(inp%D, 3, zD, dc, dn, dvv)
+    call system_clock(t1)
+    write(out_u, '(1x,a25,f11.3,a)') 'Time to get Dij matrix   ', real(t1 - t0, dp) / real(rate, dp), ' (sec)'
+    t0 = t1
+  end if
   if (inp%have_si) then
     allocate(si_par(inp%nb), si_ax(3, inp%nb))
     si_par = inp%si_par / k_B                       ! parameters.F90:958
@@ -105,7 +130,7 @@ program mc_gpu
   rank_seed = inp%seed
   if (.not. inp%have_seed) rank_seed = urandom_seed()
   seed = int(rank_seed, c_int64_t)
-  write(out_u, '(1x,a,i12)') 'Random seed                : ', rank_seed
+  call write_seed_box(out_u, inp, rank_seed, sg_used)
   call mt_seed(rng, rank_seed)
   if (inp%initial(1:5) == 'ferro') then
     spins = 0
@@ -268,7 +293,11 @@ program mc_gpu
 
   call check(esp_destroy(ctx), 'esp_destroy')
   call system_clock(t1)
-  write(out_u, '(/1x,a,f12.3,a)') 'Time for montecarlo ', real(t1 - t0, dp) / real(rate, dp), ' (sec)'
+  if (inp%pt) then                  ! mc_prog.F90:170-185
+    write(out_u, '(/1x,a,f11.3,a)') 'Time for Parallel-Tempering montecarlo ', real(t1 - t0, dp) / real(rate, dp), ' (sec)'
+  else
+    write(out_u, '(/1x,a,f11.3,a)') 'Time for montecarlo ', real(t1 - t0, dp) / real(rate, dp), ' (sec)'
+  end if
   close(out_u)
 224 format(3x,f9.4,1x,i14,1x,i14,5x,i14,7x,f10.8,4x,a)   ! pt.F90:1126
 
@@ -300,7 +329,7 @@ contains
     end do
     sgs = inp%sg_seed
     if (.not. inp%have_sg_seed) sgs = urandom_seed()
-    write(out_u, '(1x,a,i12)') 'Spin glass seed            : ', sgs
+    sg_used = sgs
     st = esp_spin_glass(int(N, c_int), zJ, jc, jn, jv, sg, sgs, bad)
     if (st /= ESP_OK) then
       write(out_u, '(2x,a)') 'Jij_parameters block is not symmetric.'

@@ -19,6 +19,8 @@ module mc_input
   type, public :: rows_t
     integer :: n = 0
     real(dp), allocatable :: f1(:,:), f2(:,:), val(:), dvec(:,:)
+    real(dp), allocatable :: dvec_frac(:,:)     ! DM vectors in fractional coordinates, as given or
+                                                ! from the unnormalised Cartesian input (parameters.F90:868-893)
     integer, allocatable :: shell(:)            ! sh= of each row (0 when absent)
     real(dp), allocatable :: sigma(:)           ! sig= of each row (eV, 0 when absent)
   end type
@@ -44,6 +46,10 @@ module mc_input
     integer :: seed = 0
     logical :: have_seed = .false., open_boundary = .false.
     logical :: energy_write = .false.     ! _energy-TNNN.dat series (parameters.F90:536-537)
+    ! units of the parameter echo only (parameters.F90:224-241): coupling values are converted to
+    ! eV on input, lengths to Angstrom
+    real(dp) :: paramconfac = 1.0_dp, lenconfac = 1.0_dp
+    logical :: coord_cart = .false.
     ! species (in order of first appearance) and the species of each sorted atom
     integer :: nsp = 0
     character(len=8), allocatable :: species(:)
@@ -575,10 +581,15 @@ contains
       if (.not. found) call input_error('no dij_vectors_frac or dij_vectors_cart block found')
       if (lines(first)(1:3) == 'ang' .or. lines(first)(1:4) == 'bohr') first = first + 1
       if (last - first + 1 /= inp%D%n) call input_error('Wrong number of lines in dij_vectors block')
-      allocate(dv(3, inp%D%n))
+      allocate(dv(3, inp%D%n), inp%D%dvec_frac(3, inp%D%n))
       do k = 1, inp%D%n
         read(lines(first + k - 1), *) dv(:, k)
-        if (frac) dv(:, k) = matmul(inp%lat, dv(:, k))
+        if (frac) then
+          inp%D%dvec_frac(:, k) = dv(:, k)
+          dv(:, k) = matmul(inp%lat, dv(:, k))
+        else
+          inp%D%dvec_frac(:, k) = matmul(transpose(inp%recip), dv(:, k)) / (2.0_dp * acos(-1.0_dp))
+        end if
         dv(:, k) = dv(:, k) / sqrt(sum(dv(:, k)**2))
       end do
       call move_alloc(dv, inp%D%dvec)
@@ -740,19 +751,23 @@ contains
         call input_error('sig in jij_parameters block must be positive')
     end subroutine
 
-    ! Keywords of the reference that only its input-generation modes (-inp1/-inp2/-ham) read,
-    ! or that change only how its output is buffered: accepted with their reference checks.
+    ! The units and coordinates of the parameter echo, and keywords of the reference that only
+    ! its input-generation modes (-inp1/-inp2/-ham) read or that change only how its output is
+    ! buffered: accepted with their reference checks.
     subroutine ignored_keywords()
       character(len=maxl) :: w
       call keyword('parameter_unit', found, w)
       if (found .and. w(1:2) /= 'ev' .and. w(1:12) /= 'electronvolt' .and. w(1:3) /= 'ryd') &
         call input_error('Value of parameter_unit not recognised in param_read')
+      if (found .and. w(1:3) == 'ryd') inp%paramconfac = 2.0_dp / hart
       call keyword('coordinate', found, w)
       if (found .and. w(1:4) /= 'frac' .and. w(1:4) /= 'cart') &
         call input_error('Value of coordinate not recognised in param_read')
+      if (found) inp%coord_cart = w(1:4) == 'cart'
       call keyword('length_unit', found, w)
       if (found .and. w(1:3) /= 'ang' .and. trim(w) /= 'bohr') &
         call input_error('Value of length_unit not recognised in param_read')
+      if (found .and. trim(w) == 'bohr') inp%lenconfac = 1.0_dp / bohr
       call keyword('neighbors_tol', found, w)
       call keyword('inputfile_setup1', found, w)
       call keyword('inputfile_setup2', found, w)

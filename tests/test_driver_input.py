@@ -55,3 +55,51 @@ def test_observable_keyword_checks(tmp_path):
     lines[k] = lines[k].split("!")[0].rstrip() + ":sig=-0.001"
     r = _run(tmp_path, "\n".join(lines) + "\n")
     assert r.returncode != 0 and "sig in jij_parameters block must be positive" in r.stdout
+
+
+def _echo_section(lines):
+    i = next(k for k, l in enumerate(lines) if l.strip() == "SYSTEM") - 2
+    j = next(k for k, l in enumerate(lines) if l.startswith(" Time to read parameters"))
+    return lines[i:j]
+
+
+@pytest.mark.parametrize("name", ["ex01_cubic", "ex02_cubic", "ex03_fef3", "ex04_bmno_pt", "ex05_snwo"])
+def test_parameter_echo_matches_published_output(tmp_path, name):
+    """The .mcout parameter echo (param_write, parameters.F90:1074-1444) of every published
+    example, line for line: lattice, reciprocal vectors, atoms, Monte Carlo and Hamiltonian
+    settings, the J/B/D tables, DM vectors, order-parameter axes and spin-correlation shells
+    (fixtures made by tests/golden/make_echo_golden.py from the examples' .mcout)."""
+    seed = name.split("_", 1)[1]
+    (tmp_path / f"{seed}.mcin").write_text(open(os.path.join(GOLDEN, "examples", f"{name}.mcin")).read())
+    r = subprocess.run([EXE, "-dryrun", seed], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    mine = _echo_section((tmp_path / f"{seed}.mcout").read_text().splitlines())
+    want = open(os.path.join(GOLDEN, "examples", f"{name}_echo.txt")).read().splitlines()
+    assert len(mine) == len(want)
+    for a, b in zip(mine, want):
+        assert a == b
+
+
+def test_parameter_echo_units_and_cartesian_rows(tmp_path):
+    """parameter_unit = ryd prints couplings times 2/hart with (Ryd) headers; length_unit = bohr
+    scales lengths by 1/bohr; coordinate = cart prints the rows' sites in Cartesian
+    coordinates, the first atom scaled by 1/bohr and the second not (parameters.F90:1206-1213)."""
+    text = open(os.path.join(GOLDEN, "examples", "ex03_fef3.mcin")).read()
+    text += "\n parameter_unit = ryd\n length_unit = bohr\n coordinate = cart\n"
+    (tmp_path / "u.mcin").write_text(text)
+    r = subprocess.run([EXE, "-dryrun", "u"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = (tmp_path / "u.mcout").read_text()
+    bohr, hart = 0.5291772108, 27.21138602
+    assert "Lattice Vectors (Bohr)" in out and "(Bohr^3)" in out and "J(Ryd)" in out and "D(Ryd)" in out
+    a1 = [float(x) for x in re.search(r"a_1\s+(\S+)\s+(\S+)\s+(\S+)", out).groups()]
+    assert a1 == pytest.approx([5.1625 / bohr, 0.0, 5.1625 / bohr], abs=1e-6)
+    assert "In Cartesian Coordinate(Bohr):" in out
+    rows = re.findall(r"^ \|\s*(\S+)\s+(\S+)\s+(\S+) \|\s*(\S+)\s+(\S+)\s+(\S+) \|\s*(\S+) \|$", out, re.M)
+    j = [r_ for r_ in rows if float(r_[6]) < 0]           # the exchange rows (J = -0.0327 eV)
+    assert len(j) == 24
+    assert float(j[0][6]) == pytest.approx(-0.0327 * 2 / hart, abs=1e-8)
+    # second row: atom 1 at the origin, atom 2 at frac (0, 0.5, 0) -> Cartesian (0, 2.58125, 2.58125) in Ang
+    assert [float(x) for x in j[1][3:6]] == pytest.approx([0.0, 2.58125, 2.58125], abs=1e-6)
+    # seventh row: atom 1 at frac (0, 0, 0.5) -> (2.58125, 2.58125, 0) Ang, printed in Bohr
+    assert [float(x) for x in j[6][0:3]] == pytest.approx([2.58125 / bohr, 2.58125 / bohr, 0.0], abs=1e-5)

@@ -355,7 +355,7 @@ def test_spin_glass_initial_energy(tmp_path):
     p.write_text(t)
     res = _run(tmp_path, "sg")
     e0 = [float(x) for x in re.findall(r"Initial Energy\s*=\s*(\S+)", res)]
-    assert "Spin glass seed" in res and "11" in res
+    assert re.search(r"Used seed for normal random generator\s*:\s*11\s*\|", res)
     inp = mcin.read(str(p))
     m = mcin.to_model(inp, builder=O.build_table_fn)
     want = O.Oracle(m).energy(Lt.ferro_spins(m.N, 1, m.mag)[0])
