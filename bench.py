@@ -236,7 +236,7 @@ def run_batch(args):
         f"{time.perf_counter() - t0:.1f} s")
     t1 = time.perf_counter()
     if args.batch_path == "auto":
-        b.autotune(nsweeps=2, cache=args.tune_cache)
+        b.autotune(nsweeps=4, cache=args.tune_cache)
     else:                     # every chain on one kernel path, one lane per site
         for k in range(len(chains)):
             b.set_chain_config(k, 1 if args.batch_path == "global" else 0, 1)

@@ -143,6 +143,9 @@ struct Chain {
   int R = 0, Ns = 0;
   std::vector<double> tems, spins;       // initial spins, reference order
   int variant = 0, tps = 1, lds = 0, lds_fits = 0, red_blocks = 1;
+  // blocks per replica of the per-step sums (k_breduce): one per 4096 per-wave entries; the
+  // direct energy's red_blocks cover 4096 spins each, 64 times the work of a per-wave sum
+  int sum_blocks = 1;
   size_t ctab_bytes = 0, lds_bytes = 0;
   // device allocations (owned)
   double *sp = nullptr, *w = nullptr, *beta = nullptr, *ebase = nullptr, *erun = nullptr,
@@ -192,7 +195,6 @@ struct esp_batch {
   std::vector<Var> vars;
   hipGraphExec_t gexec = nullptr;
   std::string gkey;
-  unsigned long long tune_sweep = 0;   // Philox sweeps used by the autotuner (its own range)
 };
 
 namespace {
@@ -313,6 +315,7 @@ int build_plan(esp_batch *b, double tilt, uint64_t seed) {
     Chain &c = b->chains[k];
     int rc0;
     if ((rc0 = size_wave_sums(b, c))) return rc0;
+    c.sum_blocks = std::max(1, std::min(c.red_blocks, (c.hm.ncol * c.mpart_nw + 4095) / 4096));
     ka[k] = chain_args(c, k);
     if (!c.lds) ka[k].mpart = c.mpart;     // spin sums only where colour phases run
     ka[k].clock = b->d_clock;
@@ -320,16 +323,16 @@ int build_plan(esp_batch *b, double tilt, uint64_t seed) {
     ka[k].key0 = (uint32_t)seed; ka[k].key1 = (uint32_t)(seed >> 32);
     lm[k] = LdsMeas{c.acc, c.erun, c.ebase, c.nmeas, 0};
     FinArgs f{};
-    f.partial = c.partial; f.nblk = c.red_blocks; f.N = c.hm.N; f.R = c.R;
+    f.partial = c.partial; f.nblk = c.sum_blocks; f.N = c.hm.N; f.R = c.R;
     f.e_base = c.ebase; f.e_run = c.erun; f.acc = c.acc; f.nmeas = c.nmeas;
     f.slot_rep = c.slot_rep; f.rep_slot = c.rep_slot; f.beta_slot = c.beta; f.nswap = c.nswap;
     f.clock = b->d_clock; f.key0 = (uint32_t)seed; f.key1 = (uint32_t)(seed >> 32);
     f.stream_base = (uint32_t)k << 16;
     fa[k] = f;
-    rb[k] = c.red_blocks;
+    rb[k] = c.sum_blocks;
     ldsf[k] = c.lds;
     parts[k] = c.partial;
-    b->max_red_blocks = std::max(b->max_red_blocks, c.red_blocks);
+    b->max_red_blocks = std::max(b->max_red_blocks, c.sum_blocks);
     if (!c.lds)
       for (int r = 0; r < c.R; ++r) red_items.push_back(make_int2(k, r));
     b->maxcol = std::max(b->maxcol, c.hm.ncol);
@@ -731,89 +734,8 @@ int esp_batch_run(esp_batch *b, int n_rounds, int steps_per_round, int steps_mea
 namespace {
 std::string shape_key(const Chain &c) {
   char k[128];
-  std::snprintf(k, sizeof k, "N%d_nb%d_z%d_col%d_v%d_R%d", c.hm.N, c.hm.nb, c.hm.zmax, c.hm.ncol,
-                c.variant, c.R);
+  std::snprintf(k, sizeof k, "N%d_nb%d_z%d_col%d_v%d", c.hm.N, c.hm.nb, c.hm.zmax, c.hm.ncol, c.variant);
   return k;
-}
-
-// Milliseconds per sweep of chain k with the given kernel shape (real Metropolis sweeps on
-// a Philox counter range no run uses, so they only add warm-up).
-int time_chain(esp_batch *b, int k, bool lds, int tps, int nsweeps, int mode, double *ms) {
-  Chain &c = b->chains[k];
-  KArgs a = chain_args(c, k);
-  a.tps = tps;
-  a.clock = b->d_clock;
-  // the trial's per-wave accepted-dE sums go to a scratch buffer laid out for its lanes per
-  // site (the chain's own state is restored after tuning); no spin sums
-  int nw = 1;
-  for (int col = 0; col < c.hm.ncol; ++col)
-    nw = std::max(nw, ((c.hm.col_off[col + 1] - c.hm.col_off[col]) * tps + 63) / 64);
-  double *wscr = nullptr;
-  a.tilt = 0.125;
-  a.key0 = 0x7475u; a.key1 = 0x6e65u;
-  int rc;
-  if ((rc = balloc(b, wscr, (size_t)c.R * c.hm.ncol * nw))) return rc;
-  BCHK(b, hipMemsetAsync(wscr, 0, sizeof(double) * c.R * c.hm.ncol * nw, b->stream));
-  a.w = wscr;
-  a.mpart = nullptr;
-  a.mpart_nw = nw;
-  KArgs *d_one = nullptr;
-  if ((rc = bupload(b, d_one, &a, 1))) return rc;
-  Clock h{(1ull << 62) + b->tune_sweep, 0, 1, 0};
-  b->tune_sweep += (unsigned long long)nsweeps;
-  BCHK(b, hipMemcpyAsync(b->d_clock, &h, sizeof(Clock), hipMemcpyHostToDevice, b->stream));
-  hipEvent_t e0, e1;
-  BCHK(b, hipEventCreate(&e0));
-  BCHK(b, hipEventCreate(&e1));
-  std::vector<int4 *> tiles(c.hm.ncol, nullptr);
-  std::vector<int> nt(c.hm.ncol, 0);
-  int2 *items = nullptr;
-  LdsMeas *d_m = nullptr;
-  if (lds) {
-    std::vector<int2> it(c.R);
-    for (int r = 0; r < c.R; ++r) it[r] = make_int2(0, r);
-    if ((rc = bupload(b, items, it.data(), it.size()))) return rc;
-    LdsMeas m{c.acc, c.erun, c.ebase, c.nmeas, 0};
-    if ((rc = bupload(b, d_m, &m, 1))) return rc;
-    if (c.lds_bytes > 65536)
-      BCHK(b, hipFuncSetAttribute((const void *)blds_fn(mode, c.variant),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds_bytes));
-  } else {
-    for (int col = 0; col < c.hm.ncol; ++col) {
-      const int cnt = c.hm.col_off[col + 1] - c.hm.col_off[col];
-      const int nblk = (cnt * tps + kBlock - 1) / kBlock;
-      std::vector<int4> t;
-      for (int bb = 0; bb < nblk; ++bb)
-        for (int r = 0; r < c.R; ++r) t.push_back(make_int4(0, r, bb, 0));
-      nt[col] = (int)t.size();
-      if ((rc = bupload(b, tiles[col], t.data(), t.size()))) return rc;
-    }
-  }
-  BCHK(b, hipEventRecord(e0, b->stream));
-  if (lds) {
-    hipLaunchKernelGGL(blds_fn(mode, c.variant), dim3(c.R), dim3(kLdsBlock), c.lds_bytes, b->stream,
-                       (const KArgs *)d_one, (const LdsMeas *)d_m, (const int2 *)items, nsweeps, 0);
-  } else {
-    for (int s = 0; s < nsweeps; ++s)
-      for (int col = 0; col < c.hm.ncol; ++col)
-        hipLaunchKernelGGL(bphase_fn(mode, c.variant), dim3(nt[col]), dim3(kBlock),
-                           phase_lds_bytes(c.variant & 3, c.hm.nb, c.hm.zmax, c.hm.rows_max), b->stream,
-                           (const KArgs *)d_one, (const int4 *)tiles[col], col, s);
-  }
-  BCHK(b, hipGetLastError());
-  BCHK(b, hipEventRecord(e1, b->stream));
-  BCHK(b, hipEventSynchronize(e1));
-  float t = 0.f;
-  BCHK(b, hipEventElapsedTime(&t, e0, e1));
-  *ms = t / nsweeps;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  for (auto *p : tiles) bfree(p);
-  bfree(items);
-  bfree(d_m);
-  bfree(d_one);
-  bfree(wscr);
-  return ESP_OK;
 }
 
 // Mean time of one round (one sweep of every chain + exchange) of the current plan, after one
@@ -890,7 +812,7 @@ void snap_free(std::vector<ChainSnap> &s) {
   }
 }
 
-int autotune_body(esp_batch *b, int nsweeps, const char *cache_path);
+int autotune_body(esp_batch *b, int nrounds, const char *cache_path);
 }  // namespace
 
 extern "C" int esp_batch_autotune(esp_batch *b, int nsweeps, const char *cache_path) {
@@ -900,7 +822,7 @@ extern "C" int esp_batch_autotune(esp_batch *b, int nsweeps, const char *cache_p
   BCHK(b, hipSetDevice(b->device));
   std::vector<ChainSnap> snap(b->chains.size());
   int rc = snap_copy(b, snap, true);
-  if (rc == ESP_OK) rc = autotune_body(b, nsweeps, cache_path);
+  if (rc == ESP_OK) rc = autotune_body(b, std::max(2, nsweeps), cache_path);
   if (rc == ESP_OK) rc = snap_copy(b, snap, false);
   (void)hipStreamSynchronize(b->stream);
   snap_free(snap);
@@ -909,103 +831,93 @@ extern "C" int esp_batch_autotune(esp_batch *b, int nsweeps, const char *cache_p
 }
 
 namespace {
-int autotune_body(esp_batch *b, int nsweeps, const char *cache_path) {
+// Whole-batch search.  Chains of one shape share a kernel configuration (global path or
+// LDS-resident, lanes per site).  Timing a shape alone favours what hides latency on an idle
+// GPU (more lanes per site, the LDS branch) although the batch as a whole is throughput-bound,
+// so every candidate is judged by the measured time of a whole round of the batch: start from
+// the throughput default (global path, one lane per site) or the cached choices, then, shape
+// by shape in order of work, keep a configuration only when it makes the round at least 0.5 %
+// faster.  Each measurement is nrounds rounds after an untimed one.  A shape is the lattice
+// and Hamiltonian (N, basis, bond width, colours, kernel variant); the replica count only
+// scales the work.
+int autotune_body(esp_batch *b, int nrounds, const char *cache_path) {
   std::map<std::string, std::pair<int, int>> decided;  // key -> (lds, tps)
-  std::map<std::string, int> lds_tps;                   // key -> best lanes per site in LDS
   if (cache_path) {
     if (FILE *f = std::fopen(cache_path, "r")) {
       char line[256], key[128];
-      int l, t, lt;
+      int l, t;
       double ms;
-      while (std::fgets(line, sizeof line, f)) {
-        const int n = std::sscanf(line, "%127s %d %d %lf %d", key, &l, &t, &ms, &lt);
-        if (n >= 4) decided[key] = {l, t};
-        if (n == 5 && lt > 0) lds_tps[key] = lt;
-      }
+      while (std::fgets(line, sizeof line, f))
+        if (std::sscanf(line, "%127s %d %d %lf", key, &l, &t, &ms) == 4 && (t == 1 || t == 2 || t == 4))
+          decided[key] = {l, t};
       std::fclose(f);
     }
   }
-  std::map<std::string, int> rep;
-  for (int k = 0; k < (int)b->chains.size(); ++k) rep.emplace(shape_key(b->chains[k]), k);
-  std::map<std::string, double> best_ms;
-  for (auto &kv : rep) {
-    const Chain &c = b->chains[kv.second];
-    auto it = decided.find(kv.first);
-    if (it != decided.end() && (!it->second.first || c.lds_fits)) continue;
-    double best = 1e300, best_lds = 1e300;
-    int bl = 0, bt = 1, blt = 0;
-    for (int lds = 0; lds < 2; ++lds) {
-      if (lds && !c.lds_fits) continue;
-      for (int tps = 1; tps <= 4; tps *= 2) {
-        if (tps > 1 && tps > c.hm.zmax) break;
-        double ms = 0;
-        int rc = time_chain(b, kv.second, lds, tps, nsweeps, ESP_MODE_RANDOM, &ms);
-        if (rc) return rc;
-        if (ms < best) { best = ms; bl = lds; bt = tps; }
-        if (lds && ms < best_lds) { best_lds = ms; blt = tps; }
-      }
-    }
-    decided[kv.first] = {bl, bt};
-    if (blt) lds_tps[kv.first] = blt;
-    best_ms[kv.first] = best;
+  // shape classes, largest share of the work first
+  std::map<std::string, std::vector<int>> cls;
+  std::map<std::string, double> work;
+  for (int k = 0; k < (int)b->chains.size(); ++k) {
+    const Chain &c = b->chains[k];
+    const std::string key = shape_key(c);
+    cls[key].push_back(k);
+    work[key] += (double)c.hm.N * c.R * c.hm.zmax;
   }
-  for (auto &c : b->chains) {
-    auto &d = decided[shape_key(c)];
-    c.lds = d.first && c.lds_fits;
-    c.tps = d.second;
+  std::vector<std::string> order;
+  for (auto &kv : cls) order.push_back(kv.first);
+  std::sort(order.begin(), order.end(), [&](const std::string &x, const std::string &y) {
+    return work[x] > work[y];
+  });
+  auto apply = [&](const std::string &key, int lds, int tps) {
+    for (int k : cls[key]) {
+      Chain &c = b->chains[k];
+      c.lds = lds && c.lds_fits;
+      c.tps = tps;
+    }
+  };
+  std::vector<std::string> todo;
+  for (auto &key : order) {
+    auto it = decided.find(key);
+    const Chain &c0 = b->chains[cls[key][0]];
+    if (it != decided.end() && (!it->second.first || c0.lds_fits) && it->second.second <= std::max(1, c0.hm.zmax)) {
+      apply(key, it->second.first, it->second.second);
+    } else {
+      apply(key, 0, 1);
+      decided[key] = {0, 1};
+      todo.push_back(key);
+    }
+  }
+  std::map<std::string, double> round_ms;
+  if (!todo.empty()) {
+    double base = 0;
+    int rc = time_rounds(b, nrounds, &base);
+    if (rc) return rc;
+    for (auto &key : todo) {
+      const Chain &c0 = b->chains[cls[key][0]];
+      for (int lds = 0; lds < 2; ++lds) {
+        if (lds && !c0.lds_fits) continue;
+        for (int tps = 1; tps <= 4; tps *= 2) {
+          if ((lds == 0 && tps == 1) || (tps > 1 && tps > c0.hm.zmax)) continue;
+          const auto prev = decided[key];
+          apply(key, lds, tps);
+          double t = 0;
+          if ((rc = time_rounds(b, nrounds, &t))) return rc;
+          if (t < 0.995 * base) {
+            base = t;
+            decided[key] = {lds, tps};
+          } else {
+            apply(key, prev.first, prev.second);
+          }
+        }
+      }
+      round_ms[key] = base;
+    }
   }
   if (cache_path) {
     if (FILE *f = std::fopen(cache_path, "w")) {
       for (auto &kv : decided)
-        std::fprintf(f, "%s %d %d %.6f %d\n", kv.first.c_str(), kv.second.first, kv.second.second,
-                     best_ms.count(kv.first) ? best_ms[kv.first] : 0.0,
-                     lds_tps.count(kv.first) ? lds_tps[kv.first] : 0);
+        std::fprintf(f, "%s %d %d %.6f\n", kv.first.c_str(), kv.second.first, kv.second.second,
+                     round_ms.count(kv.first) ? round_ms[kv.first] : 0.0);
       std::fclose(f);
-    }
-  }
-  // Branch balance.  The LDS-resident chains run beside the global-path chains (a forked
-  // graph branch), so a round lasts as long as the longer branch.  Chains that fit in LDS
-  // but ran faster alone on the global path are moved over, largest first, while the
-  // measured round time improves.
-  {
-    std::vector<int> cand;
-    for (int k = 0; k < (int)b->chains.size(); ++k)
-      if (b->chains[k].lds_fits && !b->chains[k].lds) cand.push_back(k);
-    std::sort(cand.begin(), cand.end(), [&](int x, int y) {
-      const Chain &p = b->chains[x], &q = b->chains[y];
-      return (int64_t)p.hm.N * p.R * p.hm.zmax > (int64_t)q.hm.N * q.R * q.hm.zmax;
-    });
-    if (!cand.empty()) {
-      double base = 0;
-      int rc = time_rounds(b, 3, &base);
-      if (rc) return rc;
-      size_t moved = 0, step = 1;
-      std::vector<int> old_tps(cand.size());
-      while (moved < cand.size()) {
-        const size_t m = std::min(cand.size(), moved + step);
-        for (size_t i = moved; i < m; ++i) {
-          Chain &c = b->chains[cand[i]];
-          old_tps[i] = c.tps;
-          c.lds = 1;
-          const auto it = lds_tps.find(shape_key(c));
-          c.tps = it != lds_tps.end() ? it->second : 1;
-        }
-        double t = 0;
-        if ((rc = time_rounds(b, 3, &t))) return rc;
-        if (t < 0.995 * base) {
-          base = t;
-          moved = m;
-          step *= 2;
-        } else {
-          for (size_t i = moved; i < m; ++i) {
-            Chain &c = b->chains[cand[i]];
-            c.lds = 0;
-            c.tps = old_tps[i];
-          }
-          if (step == 1) break;
-          step = 1;
-        }
-      }
     }
   }
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);

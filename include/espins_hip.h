@@ -304,9 +304,12 @@ int esp_batch_add_chain(esp_batch *b, int N, int nb, const double *mag,
 int esp_batch_set_chain_config(esp_batch *b, int chain, int path, int tps);
 /* Upload every chain (after the last esp_batch_add_chain). */
 int esp_batch_commit(esp_batch *b);
-/* Autotuner: for every distinct chain shape, time the candidate kernel shapes (path x
- * lanes per site) on this device over nsweeps sweeps and keep the fastest.  cache_path
- * (may be NULL) is a text file of earlier decisions, read first and rewritten. */
+/* Autotuner: for every distinct chain shape (largest share of the work first), try the
+ * candidate kernel shapes (global or LDS-resident path x lanes per site) and keep one only
+ * when the measured time of a whole round of the batch (nsweeps rounds, at least 2) drops by
+ * 0.5 % or more; the start is the global path with one lane per site.  The chains' state is
+ * restored afterwards.  cache_path (may be NULL) is a text file of earlier decisions, read
+ * first (shapes found there are not timed again) and rewritten. */
 int esp_batch_autotune(esp_batch *b, int nsweeps, const char *cache_path);
 /* esp_run for every chain at once (same schedule for all chains). */
 int esp_batch_run(esp_batch *b, int n_rounds, int steps_per_round, int steps_measure,
