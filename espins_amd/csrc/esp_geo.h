@@ -69,8 +69,20 @@ struct Geo {
 ESP_HD int grp_pack(int a, int r0, int r1, int r2) { return a | (r0 << 8) | (r1 << 16) | (r2 << 24); }
 
 // Per-(group, slot) record: x = first device index of the partner group (-1: padding slot,
-// which points at the zero spin N); y = q packed as (q0 + 128) | (q1 + 128) << 8 | (q2 + 128) << 16.
-ESP_HD int q_pack(int q0, int q1, int q2) { return (q0 + 128) | ((q1 + 128) << 8) | ((q2 + 128) << 16); }
+// which points at the zero spin N); y = the sub-cell shift reduced mod M, q_k in [0, M_k),
+// packed as q0 | q1 << 8 | q2 << 16 (M_k <= 255).
+ESP_HD int q_pack(int q0, int q1, int q2) { return q0 | (q1 << 8) | (q2 << 16); }
+
+// 24-bit multiply-add (full-rate v_mad_u32_u24 on the device; operands < 2^24).
+ESP_HD uint32_t umad24(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __umul24(a, b) + c;
+#else
+  return a * b + c;
+#endif
+}
+
+ESP_HD uint32_t umin32(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
 struct SiteGeo {
   int g;              // group
@@ -99,20 +111,23 @@ ESP_HD SiteGeo geo_site(const Geo &G, const int *grp, int d) {
   return s;
 }
 
-ESP_HD int wrap_add(int s, int q, int M) {
-  int v = s + q;
-  v += v < 0 ? M : 0;
-  v -= v >= M ? M : 0;
-  return v;
+// (s + q) mod M for s, q in [0, M): t = s + q < 2M, and t - M wraps to a huge unsigned value
+// when t < M, so the unsigned minimum is the reduced sum (three VALU instructions with the
+// byte extraction folded into the add).
+ESP_HD uint32_t wrap_add(uint32_t s, uint32_t q, uint32_t M) {
+  const uint32_t t = s + q;
+  return umin32(t, t - M);
 }
 
-// Device index of the partner of site s through the (group, slot) record (pg, qp).
+// Device index of the partner of site s through the (group, slot) record (pg, qp), computed
+// without branches: padding slots (pg < 0) select the zero spin N.
 ESP_HD int geo_partner(const Geo &G, const SiteGeo &s, int pg, int qp) {
-  const int v0 = wrap_add(s.s0, (qp & 0xff) - 128, G.M0);
-  const int v1 = wrap_add(s.s1, ((qp >> 8) & 0xff) - 128, G.M1);
-  const int v2 = wrap_add(s.s2, ((qp >> 16) & 0xff) - 128, G.M2);
-  const int j = pg + (v2 * G.M1 + v1) * G.M0 + v0;
-  return pg < 0 ? G.N : j;
+  const uint32_t uq = (uint32_t)qp;
+  const uint32_t v0 = wrap_add((uint32_t)s.s0, uq & 0xffu, (uint32_t)G.M0);
+  const uint32_t v1 = wrap_add((uint32_t)s.s1, (uq >> 8) & 0xffu, (uint32_t)G.M1);
+  const uint32_t v2 = wrap_add((uint32_t)s.s2, (uq >> 16) & 0xffu, (uint32_t)G.M2);
+  const uint32_t j = (uint32_t)pg + umad24(umad24(v2, (uint32_t)G.M1, v1), (uint32_t)G.M0, v0);
+  return pg < 0 ? G.N : (int)j;
 }
 
 }  // namespace espg

@@ -357,7 +357,7 @@ void build_implicit(HostModel &hm, const int L[3], const int P[3], const double 
   for (int k = 0; k < 3; ++k) {
     if (P[k] < 1 || L[k] % P[k] != 0 || P[k] > 255) return;
     M[k] = L[k] / P[k];
-    if (M[k] > 254) return;
+    if (M[k] > 255) return;
   }
   const long mt = (long)M[0] * M[1] * M[2];
   if (mt <= 0 || N % mt != 0) return;
@@ -392,9 +392,7 @@ void build_implicit(HostModel &hm, const int L[3], const int P[3], const double 
       cell(j, y);
       for (int k = 0; k < 3; ++k) {
         const int s_i = x[k] / P[k], s_j = y[k] / P[k];
-        int v = ((s_j - s_i) % M[k] + M[k]) % M[k];
-        if (v > M[k] / 2) v -= M[k];
-        q[k] = v;
+        q[k] = ((s_j - s_i) % M[k] + M[k]) % M[k];   // in [0, M)
       }
       e[0] = (int)((jd / mt) * mt);
       e[1] = espg::q_pack(q[0], q[1], q[2]);
