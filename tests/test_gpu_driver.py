@@ -84,6 +84,25 @@ def test_example02_parallel_tempering(tmp_path):
     perm = [int(x) for x in m[-1].split()]
     assert sorted(perm) == list(range(1, 33))
     assert "swaps" in out
+    # per temperature slot: running energy = direct energy, and the direct energy is the
+    # oracle's energy of the configuration written for that slot (3F12.6 spins, E15.7 print)
+    adds = [float(x) for x in re.findall(r"adding differences\s*=\s*(\S+)\s+<-- ENERGY3", out)]
+    direct = [float(x) for x in re.findall(r"direct calculations\s*=\s*(\S+)\s+<-- ENERGY3", out)]
+    assert len(adds) == len(direct) == 32
+    assert np.allclose(adds, direct, rtol=1e-6)
+    from espins_amd import mcin
+    from oracle import oracle as O
+    model = mcin.to_model(mcin.read(str(tmp_path / "cubic.mcin")), builder=O.build_table_fn)
+    orc = O.Oracle(model)
+    lines = (tmp_path / "cubic_sconfig.dat").read_text().splitlines()
+    N, R = (int(v) for v in lines[2].split())
+    assert (N, R) == (model.N, 32)
+    p = 3
+    for t in range(R):
+        assert float(lines[p]) == pytest.approx(mine[t, 0], abs=1e-4)
+        spins = np.array([[float(v) for v in lines[p + 1 + i].split()] for i in range(N)])
+        p += N + 1
+        assert orc.energy(spins) == pytest.approx(direct[t], rel=1e-4)
 
 
 def test_example01_learned_proposal(tmp_path):
