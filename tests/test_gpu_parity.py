@@ -154,6 +154,39 @@ def test_philox_sweep_matches_oracle(En, name, m, mode, path):
         assert st["e_run"][r] == pytest.approx(e, rel=1e-10, abs=1e-8)
 
 
+@pytest.mark.parametrize("name", ["C2 FCC 16^3 J1-J3 Delta", "C4 SC 64^3 J1 B DM Delta", "C5 FCC 32^3 J1-J2 B Delta"])
+def test_full_size_philox_sweep_matches_oracle(En, name):
+    """The benchmark Hamiltonians at their full sizes: two on-device sweeps ('random', then
+    'const') of a cold and a hot replica give the oracle's colour-order Philox sweep --
+    the same rejections and spins to 1e-12."""
+    if name.startswith("C2"):
+        m, _ = Lt.config_c2(L=16, R=2)
+    elif name.startswith("C4"):
+        m, _ = Lt.config_c4(L=64, R=2)
+    else:
+        m, _ = Lt.config_c5(L=32, R=2)
+    tems = np.array([0.1, 2.0])
+    spins = Lt.random_spins(m.N, 2, 23, m.mag)
+    seed = 0x5A5A5A5A
+    with En.Engine() as eng:
+        eng.set_model(m).set_replicas(tems, spins)
+        off, sites, _ = colour_order(eng)
+        say("full-size sweep", name, m.N, eng.model_info())
+        eng.sweep(1, mode=0, seed=seed, sweep0=5)
+        eng.sweep(1, mode=1, tilt=0.2, seed=seed, sweep0=6)
+        got = eng.spins()
+        st = eng.stats()
+    o = O.Oracle(m)
+    for r in range(2):
+        s = spins[r].copy()
+        e = o.energy(s)
+        e, n0 = o.sweep_colour_philox(s, off, sites, 1.0 / tems[r], 0, 0.2, seed, 5, r, e)
+        e, n1 = o.sweep_colour_philox(s, off, sites, 1.0 / tems[r], 1, 0.2, seed, 6, r, e)
+        assert st["rejected"][r] == n0 + n1, (r, st["rejected"][r], n0, n1)
+        np.testing.assert_allclose(got[r], s, rtol=0, atol=1e-12)
+        assert st["e_run"][r] == pytest.approx(e, rel=1e-10, abs=1e-8)
+
+
 def test_pt_exchange_matches_oracle(En):
     m = Lt.cubic_model("sc", 4, J=(1.0,))
     R = 9
