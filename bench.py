@@ -243,8 +243,9 @@ def run_batch(args):
     shapes = {}
     for k in range(len(chains)):
         i = b.info(k)
-        shapes[(i["lds"], i["tps"])] = shapes.get((i["lds"], i["tps"]), 0) + 1
-    log(f"autotune: {time.perf_counter() - t1:.1f} s, (lds, lanes) -> chains {shapes}")
+        key = (i["lds"], i["tps"], i["chunk"])
+        shapes[key] = shapes.get(key, 0) + 1
+    log(f"autotune: {time.perf_counter() - t1:.1f} s, (lds, lanes, chunk) -> chains {shapes}")
     graph = not args.no_graph
     b.run(args.warmup, measure=True, exchange=True, seed=7, graph=graph)
     b.reset_stats()

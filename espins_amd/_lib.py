@@ -115,6 +115,8 @@ _SIGS.update({
                                  _i, _ip, _ip, _dp, _i, _dp, _dp, _i, _dp, _dp,
                                  _ip, _i, _dp, _dp, _ip]),
     "esp_batch_set_chain_config": (_i, [_p, _i, _i, _i]),
+    "esp_batch_set_chain_chunk": (_i, [_p, _i, _i]),
+    "esp_batch_get_chain_chunk": (_i, [_p, _i, _ip]),
     "esp_batch_commit": (_i, [_p]),
     "esp_batch_autotune": (_i, [_p, _i, C.c_char_p]),
     "esp_batch_run": (_i, [_p, _i, _i, _i, _i, _i, _i, _i, _d, _u64, _u64, _u64, _i]),

@@ -27,7 +27,7 @@ namespace {
 
 // ---- batched kernels -------------------------------------------------------------------
 // tiles: (chain, replica, block, 0); chain < 0 = padding tile (XCD interleave)
-template <int MODE, int LAY, bool HB, bool HD, bool SELF>
+template <int MODE, int LAY, bool HB, bool HD, bool SELF, int CH>
 __global__ __launch_bounds__(kBlock) ESP_PHASE_ATTR void k_bphase(const KArgs *chains, const int4 *tiles, int colour,
                                                    int sweep_off) {
   extern __shared__ double s_ctab[];
@@ -38,7 +38,7 @@ __global__ __launch_bounds__(kBlock) ESP_PHASE_ATTR void k_bphase(const KArgs *c
   a.c_begin = gldi(a.col_off, colour);
   a.c_count = gldi(a.col_off, colour + 1) - a.c_begin;
   a.sweep_off = sweep_off;
-  phase_body<MODE, LAY, HB, HD, SELF>(a, t.y, t.z, s_ctab);
+  phase_body<MODE, LAY, HB, HD, SELF, CH>(a, t.y, t.z, s_ctab);
 }
 
 // LDS-resident chains: one block per (chain, replica)
@@ -91,20 +91,26 @@ __global__ __launch_bounds__(1024) void k_bfinal(const FinArgs *fa, Clock *clock
 using BPhaseFn = void (*)(const KArgs *, const int4 *, int, int);
 using BLdsFn = void (*)(const KArgs *, const LdsMeas *, const int2 *, int, int);
 
-// variant bits: 0-1 coefficient layout (LAY_*), 2 biquadratic, 3 DM, 4 self bonds
-template <int MODE, int LAY, bool HB, bool HD>
-BPhaseFn bp4(bool self) { return self ? k_bphase<MODE, LAY, HB, HD, true> : k_bphase<MODE, LAY, HB, HD, false>; }
-template <int MODE, int LAY, bool HB>
-BPhaseFn bp3(bool hd, bool self) { return hd ? bp4<MODE, LAY, HB, true>(self) : bp4<MODE, LAY, HB, false>(self); }
-template <int MODE, int LAY>
-BPhaseFn bp2(bool hb, bool hd, bool self) { return hb ? bp3<MODE, LAY, true>(hd, self) : bp3<MODE, LAY, false>(hd, self); }
-template <int MODE>
+// variant bits: 0-1 coefficient layout (LAY_*), 2 biquadratic, 3 DM, 4 self bonds; launch
+// variants add bit 5: partner gathers in flight per lane 4 instead of 2 (global path)
+constexpr int kVarChunk4 = 32;
+template <int MODE, int LAY, bool HB, bool HD, int CH>
+BPhaseFn bp4(bool self) { return self ? k_bphase<MODE, LAY, HB, HD, true, CH> : k_bphase<MODE, LAY, HB, HD, false, CH>; }
+template <int MODE, int LAY, bool HB, int CH>
+BPhaseFn bp3(bool hd, bool self) { return hd ? bp4<MODE, LAY, HB, true, CH>(self) : bp4<MODE, LAY, HB, false, CH>(self); }
+template <int MODE, int LAY, int CH>
+BPhaseFn bp2(bool hb, bool hd, bool self) {
+  return hb ? bp3<MODE, LAY, true, CH>(hd, self) : bp3<MODE, LAY, false, CH>(hd, self);
+}
+template <int MODE, int CH>
 BPhaseFn bp1(int v) {
   const int lay = v & 3;
-  if (lay == LAY_IMP) return bp2<MODE, LAY_IMP>(v & 4, v & 8, v & 16);
-  return lay == LAY_CMP ? bp2<MODE, LAY_CMP>(v & 4, v & 8, v & 16) : bp2<MODE, LAY_BOND>(v & 4, v & 8, v & 16);
+  if (lay == LAY_IMP) return bp2<MODE, LAY_IMP, CH>(v & 4, v & 8, v & 16);
+  return lay == LAY_CMP ? bp2<MODE, LAY_CMP, CH>(v & 4, v & 8, v & 16) : bp2<MODE, LAY_BOND, CH>(v & 4, v & 8, v & 16);
 }
-BPhaseFn bphase_fn(int mode, int variant) { return mode == M_RANDOM ? bp1<M_RANDOM>(variant) : bp1<M_CONST>(variant); }
+template <int MODE>
+BPhaseFn bp0(int v) { return (v & kVarChunk4) ? bp1<MODE, 4>(v) : bp1<MODE, kChunk>(v); }
+BPhaseFn bphase_fn(int mode, int variant) { return mode == M_RANDOM ? bp0<M_RANDOM>(variant) : bp0<M_CONST>(variant); }
 
 template <int MODE, bool CMP, bool HB, bool HD>
 BLdsFn bl4(bool self) { return self ? k_blds<MODE, CMP, HB, HD, true> : k_blds<MODE, CMP, HB, HD, false>; }
@@ -143,6 +149,7 @@ struct Chain {
   int R = 0, Ns = 0;
   std::vector<double> tems, spins;       // initial spins, reference order
   int variant = 0, tps = 1, lds = 0, lds_fits = 0, red_blocks = 1;
+  int chunk = kChunk;                    // partner gathers in flight per lane (global path): 2 or 4
   // blocks per replica of the per-step sums (k_breduce): one per 4096 per-wave entries; the
   // direct energy's red_blocks cover 4096 spins each, 64 times the work of a per-wave sum
   int sum_blocks = 1;
@@ -160,6 +167,10 @@ struct Chain {
   int *grp = nullptr, *gslot = nullptr;   // implicit geometry (esp_geo.h)
   double *magb = nullptr;
 };
+
+// Kernel grouping of a chain's colour phases: its Hamiltonian variant and, on the global path,
+// the chunk width.
+int launch_variant(const Chain &c) { return c.variant | (!c.lds && c.chunk == 4 ? kVarChunk4 : 0); }
 
 }  // namespace
 
@@ -346,13 +357,13 @@ int build_plan(esp_batch *b, double tilt, uint64_t seed) {
   if ((rc = bupload(b, b->d_partials, parts.data(), nc))) return rc;
   if ((rc = bupload(b, b->d_red_items, red_items.data(), red_items.size()))) return rc;
   b->n_red_items = (int)red_items.size();
-  // variants
+  // launch variants: the kernel variant and, for global-path chains, the chunk width
   std::map<int, int> vid;
   for (auto &c : b->chains)
-    if (!vid.count(c.variant)) {
-      vid[c.variant] = (int)b->vars.size();
+    if (!vid.count(launch_variant(c))) {
+      vid[launch_variant(c)] = (int)b->vars.size();
       b->vars.emplace_back();
-      b->vars.back().variant = c.variant;
+      b->vars.back().variant = launch_variant(c);
     }
   for (auto &v : b->vars) {
     // global-path tiles per colour, interleaved so that tile t runs on XCD t % 8 and all
@@ -364,7 +375,7 @@ int build_plan(esp_batch *b, double tilt, uint64_t seed) {
       int qi = 0;
       for (int k = 0; k < nc; ++k) {
         const Chain &c = b->chains[k];
-        if (c.variant != v.variant || c.lds || col >= c.hm.ncol) continue;
+        if (launch_variant(c) != v.variant || c.lds || col >= c.hm.ncol) continue;
         const int cnt = c.hm.col_off[col + 1] - c.hm.col_off[col];
         const int nblk = (cnt * c.tps + kBlock - 1) / kBlock;
         // Implicit geometry: every group of the colour has the same sub-cell layout, so the
@@ -398,7 +409,7 @@ int build_plan(esp_batch *b, double tilt, uint64_t seed) {
     size_t shm_l = 0, shm_p = 0;
     for (int k = 0; k < nc; ++k) {
       const Chain &c = b->chains[k];
-      if (c.variant != v.variant) continue;
+      if (launch_variant(c) != v.variant) continue;
       shm_p = std::max(shm_p, phase_lds_bytes(c.variant & 3, c.hm.nb, c.hm.zmax, c.hm.rows_max));
       if (!c.lds) continue;
       shm_l = std::max(shm_l, c.lds_bytes);
@@ -410,7 +421,7 @@ int build_plan(esp_batch *b, double tilt, uint64_t seed) {
     if (!li.empty() && (rc = bupload(b, v.d_lds, li.data(), li.size()))) return rc;
     if (shm_l > 65536)
       for (int mode = 0; mode < 2; ++mode)
-        BCHK(b, hipFuncSetAttribute((const void *)blds_fn(mode, v.variant),
+        BCHK(b, hipFuncSetAttribute((const void *)blds_fn(mode, v.variant & (kVarChunk4 - 1)),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_l));
   }
   BCHK(b, hipStreamSynchronize(b->stream));
@@ -426,7 +437,7 @@ int enqueue_round(esp_batch *b, int steps, int M, bool measure, bool exchange, i
     BCHK(b, hipStreamWaitEvent(b->side, b->fork, 0));
     for (auto &v : b->vars)
       if (v.nlds > 0)
-        hipLaunchKernelGGL(blds_fn(mode, v.variant), dim3(v.nlds), dim3(kLdsBlock), v.shm_lds, b->side,
+        hipLaunchKernelGGL(blds_fn(mode, v.variant & (kVarChunk4 - 1)), dim3(v.nlds), dim3(kLdsBlock), v.shm_lds, b->side,
                            (const KArgs *)b->d_chains, (const LdsMeas *)b->d_meas,
                            (const int2 *)v.d_lds, steps * M, measure ? M : 0);
     BCHK(b, hipGetLastError());
@@ -600,6 +611,22 @@ int esp_batch_set_chain_config(esp_batch *b, int k, int path, int tps) {
   b->gkey.clear();
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   b->gexec = nullptr;
+  return ESP_OK;
+}
+
+int esp_batch_set_chain_chunk(esp_batch *b, int k, int chunk) {
+  if (!b || k < 0 || k >= (int)b->chains.size()) return ESP_EINVAL;
+  if (chunk != 2 && chunk != 4) return bfail(b, ESP_EINVAL, "chunk must be 2 or 4");
+  b->chains[k].chunk = chunk;
+  b->gkey.clear();
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+  return ESP_OK;
+}
+
+int esp_batch_get_chain_chunk(esp_batch *b, int k, int *chunk) {
+  if (!b || !chunk || k < 0 || k >= (int)b->chains.size()) return ESP_EINVAL;
+  *chunk = b->chains[k].chunk;
   return ESP_OK;
 }
 
@@ -832,8 +859,9 @@ extern "C" int esp_batch_autotune(esp_batch *b, int nsweeps, const char *cache_p
 
 namespace {
 // Whole-batch search.  Chains of one shape share a kernel configuration (global path or
-// LDS-resident, lanes per site).  Timing a shape alone favours what hides latency on an idle
-// GPU (more lanes per site, the LDS branch) although the batch as a whole is throughput-bound,
+// LDS-resident, lanes per site, partner gathers in flight per lane).  Timing a shape alone
+// favours what hides latency on an idle GPU (more lanes per site, the LDS branch) although
+// the batch as a whole is throughput-bound,
 // so every candidate is judged by the measured time of a whole round of the batch: start from
 // the throughput default (global path, one lane per site) or the cached choices, then, shape
 // by shape in order of work, keep a configuration only when it makes the round at least 0.5 %
@@ -841,15 +869,20 @@ namespace {
 // and Hamiltonian (N, basis, bond width, colours, kernel variant); the replica count only
 // scales the work.
 int autotune_body(esp_batch *b, int nrounds, const char *cache_path) {
-  std::map<std::string, std::pair<int, int>> decided;  // key -> (lds, tps)
+  struct Shape {
+    int lds = 0, tps = 1, chunk = kChunk;
+  };
+  std::map<std::string, Shape> decided;
   if (cache_path) {
     if (FILE *f = std::fopen(cache_path, "r")) {
       char line[256], key[128];
-      int l, t;
+      int l, t, ch;
       double ms;
-      while (std::fgets(line, sizeof line, f))
-        if (std::sscanf(line, "%127s %d %d %lf", key, &l, &t, &ms) == 4 && (t == 1 || t == 2 || t == 4))
-          decided[key] = {l, t};
+      while (std::fgets(line, sizeof line, f)) {
+        const int n = std::sscanf(line, "%127s %d %d %lf %d", key, &l, &t, &ms, &ch);
+        if (n >= 4 && (t == 1 || t == 2 || t == 4))
+          decided[key] = Shape{l, t, n == 5 && (ch == 2 || ch == 4) ? ch : kChunk};
+      }
       std::fclose(f);
     }
   }
@@ -867,22 +900,23 @@ int autotune_body(esp_batch *b, int nrounds, const char *cache_path) {
   std::sort(order.begin(), order.end(), [&](const std::string &x, const std::string &y) {
     return work[x] > work[y];
   });
-  auto apply = [&](const std::string &key, int lds, int tps) {
+  auto apply = [&](const std::string &key, const Shape &sh) {
     for (int k : cls[key]) {
       Chain &c = b->chains[k];
-      c.lds = lds && c.lds_fits;
-      c.tps = tps;
+      c.lds = sh.lds && c.lds_fits;
+      c.tps = sh.tps;
+      c.chunk = sh.chunk;
     }
   };
   std::vector<std::string> todo;
   for (auto &key : order) {
     auto it = decided.find(key);
     const Chain &c0 = b->chains[cls[key][0]];
-    if (it != decided.end() && (!it->second.first || c0.lds_fits) && it->second.second <= std::max(1, c0.hm.zmax)) {
-      apply(key, it->second.first, it->second.second);
+    if (it != decided.end() && (!it->second.lds || c0.lds_fits) && it->second.tps <= std::max(1, c0.hm.zmax)) {
+      apply(key, it->second);
     } else {
-      apply(key, 0, 1);
-      decided[key] = {0, 1};
+      decided[key] = Shape{};
+      apply(key, decided[key]);
       todo.push_back(key);
     }
   }
@@ -893,20 +927,23 @@ int autotune_body(esp_batch *b, int nrounds, const char *cache_path) {
     if (rc) return rc;
     for (auto &key : todo) {
       const Chain &c0 = b->chains[cls[key][0]];
-      for (int lds = 0; lds < 2; ++lds) {
-        if (lds && !c0.lds_fits) continue;
-        for (int tps = 1; tps <= 4; tps *= 2) {
-          if ((lds == 0 && tps == 1) || (tps > 1 && tps > c0.hm.zmax)) continue;
-          const auto prev = decided[key];
-          apply(key, lds, tps);
-          double t = 0;
-          if ((rc = time_rounds(b, nrounds, &t))) return rc;
-          if (t < 0.995 * base) {
-            base = t;
-            decided[key] = {lds, tps};
-          } else {
-            apply(key, prev.first, prev.second);
-          }
+      std::vector<Shape> cand;
+      for (int tps = 1; tps <= 4; tps *= 2) {
+        if (tps > 1 && tps > c0.hm.zmax) break;
+        for (int ch = 2; ch <= 4; ch += 2)
+          if (tps > 1 || ch != kChunk) cand.push_back(Shape{0, tps, ch});
+        if (c0.lds_fits) cand.push_back(Shape{1, tps, kChunk});
+      }
+      for (const Shape &sh : cand) {
+        const Shape prev = decided[key];
+        apply(key, sh);
+        double t = 0;
+        if ((rc = time_rounds(b, nrounds, &t))) return rc;
+        if (t < 0.995 * base) {
+          base = t;
+          decided[key] = sh;
+        } else {
+          apply(key, prev);
         }
       }
       round_ms[key] = base;
@@ -915,8 +952,8 @@ int autotune_body(esp_batch *b, int nrounds, const char *cache_path) {
   if (cache_path) {
     if (FILE *f = std::fopen(cache_path, "w")) {
       for (auto &kv : decided)
-        std::fprintf(f, "%s %d %d %.6f\n", kv.first.c_str(), kv.second.first, kv.second.second,
-                     round_ms.count(kv.first) ? round_ms[kv.first] : 0.0);
+        std::fprintf(f, "%s %d %d %.6f %d\n", kv.first.c_str(), kv.second.lds, kv.second.tps,
+                     round_ms.count(kv.first) ? round_ms[kv.first] : 0.0, kv.second.chunk);
       std::fclose(f);
     }
   }

@@ -542,11 +542,18 @@ class Batch:
                                               C.byref(upd)))
         return dict(kernel_ms=kms.value, nlaunch=nl.value, wall_ms=wall.value, updates=upd.value)
 
+    def set_chain_chunk(self, k, chunk):
+        """Partner gathers in flight per lane on chain k's global-memory colour phases (2 or 4)."""
+        self._chk(E.lib.esp_batch_set_chain_chunk(self._h, int(k), int(chunk)))
+
     def info(self, k):
         out = np.zeros(8, np.int32)
         self._chk(E.lib.esp_batch_info(self._h, int(k), E.ptr(out, C.c_int)))
+        ch = C.c_int()
+        self._chk(E.lib.esp_batch_get_chain_chunk(self._h, int(k), C.byref(ch)))
         return dict(N=int(out[0]), R=int(out[1]), ncol=int(out[2]), tps=int(out[3]),
-                    lds=bool(out[4]), zmax=int(out[5]), variant=int(out[6]), lds_fits=bool(out[7]))
+                    lds=bool(out[4]), zmax=int(out[5]), variant=int(out[6]), lds_fits=bool(out[7]),
+                    chunk=int(ch.value))
 
     def stats(self, k):
         N, R, _ = self.chains[k]

@@ -302,10 +302,16 @@ int esp_batch_add_chain(esp_batch *b, int N, int nb, const double *mag,
                         int *chain_id);
 /* Per-chain kernel shape: path 0 auto / 1 global / 2 LDS, tps lanes per site. */
 int esp_batch_set_chain_config(esp_batch *b, int chain, int path, int tps);
+/* Partner gathers in flight per lane on chain k's global-memory colour phases: 2 (default)
+ * or 4 (more memory-level parallelism per wave, more registers).  A kernel shape with no
+ * counterpart in the reference; the autotuner searches it. */
+int esp_batch_set_chain_chunk(esp_batch *b, int chain, int chunk);
+int esp_batch_get_chain_chunk(esp_batch *b, int chain, int *chunk);
 /* Upload every chain (after the last esp_batch_add_chain). */
 int esp_batch_commit(esp_batch *b);
 /* Autotuner: for every distinct chain shape (largest share of the work first), try the
- * candidate kernel shapes (global or LDS-resident path x lanes per site) and keep one only
+ * candidate kernel shapes (global or LDS-resident path x lanes per site x partner gathers in
+ * flight per lane) and keep one only
  * when the measured time of a whole round of the batch (nsweeps rounds, at least 2) drops by
  * 0.5 % or more; the start is the global path with one lane per site.  The chains' state is
  * restored afterwards.  cache_path (may be NULL) is a text file of earlier decisions, read

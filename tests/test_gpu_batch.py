@@ -88,7 +88,7 @@ def test_autotune_picks_valid_shapes(tmp_path):
         b.autotune(nsweeps=3, cache=cache)
         for k in range(len(cs)):
             i = b.info(k)
-            assert i["tps"] in (1, 2, 4)
+            assert i["tps"] in (1, 2, 4) and i["chunk"] in (2, 4)
             assert (not i["lds"]) or i["lds_fits"]
         b.run(3, measure=True, exchange=True, seed=1)
         for k in range(len(cs)):
@@ -133,3 +133,30 @@ def test_autotune_leaves_chain_state_unchanged(tmp_path):
         for x, y in zip(runs[0], other):
             assert np.array_equal(x[0], y[0])
             assert np.array_equal(x[1][0], y[1][0])
+
+
+def test_chunk_width_changes_nothing_but_speed():
+    """Four partner gathers in flight per lane instead of two: the slots are still summed in
+    order, so every chain's spins, rejections and labels are bit-identical; the estimators
+    agree to 1e-12 (the compiler may contract the bond sums of the two kernel instances into
+    different fused multiply-adds, which moves dE by an ulp)."""
+    from espins_amd.engine import Batch
+    cs = chains()
+    init = [Lt.random_spins(m.N, len(t), 70 + k, m.mag) for k, (m, t) in enumerate(cs)]
+    res = []
+    for chunk in (2, 4):
+        with Batch() as b:
+            for (m, t), s in zip(cs, init):
+                b.add_chain(m, t, s)
+            for k in range(len(cs)):
+                b.set_chain_config(k, 1, 1)
+                b.set_chain_chunk(k, chunk)
+                assert b.info(k)["chunk"] == chunk
+            b.commit()
+            b.run(3, measure=True, exchange=True, seed=9)
+            res.append([(b.spins(k), b.stats(k), b.pt_map(k)) for k in range(len(cs))])
+    for (s2, st2, m2), (s4, st4, m4) in zip(*res):
+        assert np.array_equal(s2, s4)
+        assert np.array_equal(st2["rejected"], st4["rejected"])
+        np.testing.assert_allclose(st2["acc"], st4["acc"], rtol=1e-12)
+        assert np.array_equal(m2[0], m4[0])
