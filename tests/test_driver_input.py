@@ -103,3 +103,25 @@ def test_parameter_echo_units_and_cartesian_rows(tmp_path):
     assert [float(x) for x in j[1][3:6]] == pytest.approx([0.0, 2.58125, 2.58125], abs=1e-6)
     # seventh row: atom 1 at frac (0, 0, 0.5) -> (2.58125, 2.58125, 0) Ang, printed in Bohr
     assert [float(x) for x in j[6][0:3]] == pytest.approx([2.58125 / bohr, 2.58125 / bohr, 0.0], abs=1e-5)
+
+
+def test_parameter_echo_spin_glass_sigma_table(tmp_path):
+    """With spin_glass the echo prints the Sigma table (one row per Jij row, sig= in eV) in
+    place of the shell table (parameters.F90:1217-1247)."""
+    text = open(os.path.join(GOLDEN, "examples", "ex01_cubic.mcin")).read()
+    rows = []
+    for l in text.splitlines():
+        if "jij=" in l.replace(" ", "").lower() and l.strip().lower().startswith(("f1", "c1")):
+            l = l.split("!")[0].rstrip() + ":sig=0.003"
+        rows.append(l)
+    text = "\n".join(rows) + "\n spin_glass = .true.\n spin_glass_seed = 11\n"
+    (tmp_path / "sg.mcin").write_text(text)
+    r = subprocess.run([EXE, "-dryrun", "sg"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = (tmp_path / "sg.mcout").read_text()
+    assert re.search(r"Using random exchange parameters\s*:\s*T", out)
+    sigma = out[out.index("-- Sigma --"):]
+    assert "Sigma(eV)" in sigma.splitlines()[3]
+    vals = re.findall(r"\|\s*0\.00300000 \|$", sigma, re.M)
+    nj = len(re.findall(r"\|\s*-?\d+\.\d{8} \|$", out[out.index("-- Exchange --"):out.index("-- Sigma --")], re.M))
+    assert nj > 0 and len(vals) == nj
