@@ -331,7 +331,7 @@ def profile_ess():
     got = {}
     for line in open(path):
         t = line.strip()
-        for key in ("ess_ratio_vs_random", "ess_ratio_vs_const"):
+        for key in ("ess_ratio_vs_random", "ess_ratio_vs_const", "ess_ratio_M_vs_random", "ess_ratio_M_vs_const"):
             if t.startswith(key + " ="):
                 got[key] = ast.literal_eval(t.split("=", 1)[1].strip())
     if not got:
@@ -421,22 +421,27 @@ def run_learned(args):
         pt(args.steps, mode, sweep0 + args.ess_warmup, tilt)
         eng.sync()
         dt = time.perf_counter() - t2
-        tr = eng.trace()[:, :, 0]
+        trace = eng.trace()
+        tr = trace[:, :, 0]                       # E per measurement and slot
         st = eng.stats()
         eng.set_trace(0)
         ess, tau = Ln.ess_per_slot(tr)
+        ess_m, tau_m = Ln.ess_per_slot(trace[:, :, 1])   # |m|
         acc = 1.0 - st["rejected"] / float(m.N * args.steps)
         h = args.steps // 2       # drift: first-half minus second-half mean, in standard errors
         se = np.sqrt(2 * tau / h) * tr.std(0)
         drift = (tr[:h].mean(0) - tr[h:].mean(0)) / np.maximum(se * np.sqrt(2), 1e-300)
-        res[mode] = dict(ess_s=ess / dt, tau=tau, dt=dt, acc=acc, e=tr.mean(0), drift=drift)
+        res[mode] = dict(ess_s=ess / dt, tau=tau, dt=dt, acc=acc, e=tr.mean(0), drift=drift,
+                         ess_m_s=ess_m / dt, tau_m=tau_m)
         log(f"{mode:8s}: {args.steps} steps in {dt:.2f} s ({m.N * R * args.steps / dt / 1e9:.3f} G updates/s), "
             f"acceptance {acc.min():.4f}-{acc.max():.4f}, tau_int(E) median {np.median(tau):.1f} "
             f"max {tau.max():.1f} sweeps, ESS/s total {ess.sum() / dt:.1f}, "
+            f"tau_int(|m|) median {np.median(tau_m):.1f}, ESS/s of |m| total {ess_m.sum() / dt:.1f}, "
             f"half-to-half drift max {np.abs(drift).max():.1f} sigma")
         log("   tau by slot (cold -> hot): " + " ".join(f"{x:.0f}" for x in tau[::4]))
     lo = slice(0, R // 4)                       # coldest quarter of the ladder
     ratio = {k: res["learned"]["ess_s"] / res[k]["ess_s"] for k in ("random", "const")}
+    ratio_m = {k: res["learned"]["ess_m_s"] / res[k]["ess_m_s"] for k in ("random", "const")}
     reliable = int(np.sum(res["random"]["tau"] * 50 < args.steps))
     tm = eng.time_sweeps(20, mode="learned", seed=seed + 1, sweep0=10 ** 9)
     B = phase_bytes(m)[0] / m.N                  # minimal bytes per update, mean over colours
@@ -458,7 +463,14 @@ def run_learned(args):
         "ess_ratio_vs_const": {"median": float(np.median(ratio["const"])),
                                "coldest_quarter": float(np.median(ratio["const"][lo])),
                                "min": float(ratio["const"].min())},
+        "ess_ratio_M_vs_random": {"median": float(np.median(ratio_m["random"])),
+                                  "coldest_quarter": float(np.median(ratio_m["random"][lo])),
+                                  "min": float(ratio_m["random"].min())},
+        "ess_ratio_M_vs_const": {"median": float(np.median(ratio_m["const"])),
+                                 "coldest_quarter": float(np.median(ratio_m["const"][lo])),
+                                 "min": float(ratio_m["const"].min())},
         "tau_int_E": {k: [float(x) for x in v["tau"]] for k, v in res.items()},
+        "tau_int_M": {k: [float(x) for x in v["tau_m"]] for k, v in res.items()},
         "random_tau_reliable_slots": reliable,
         "drift_sigma_max": {k: float(np.abs(v["drift"]).max()) for k, v in res.items()},
         "updates_per_s": {k: m.N * R * args.steps / v["dt"] for k, v in res.items()},

@@ -10,7 +10,7 @@
 // neighbours, which a colour phase never changes, so detailed balance holds for any weights:
 // the network only decides how often moves are accepted.
 //
-// Network: 5 -> 32 -> 32 -> 32 -> 6, hard-swish, BF16 weights for sampling (FP32 accumulate), FP32
+// Network: 5 -> 32 -> 32 -> 32 -> 6, GELU (tanh form), BF16 weights for sampling (FP32 accumulate), FP32
 // master weights for training with Adam.  Outputs per component k: log-scale s_k of the
 // concentration kappa_k = beta|G| exp(s_k), weight logit, tilt t_k = (pi/2) tanh(.) of the
 // mean direction from G-hat toward the nearer pole of the anisotropy axis.
@@ -42,14 +42,21 @@ constexpr double kPi = 3.14159265358979323846;
 // at most a factor kUniform against uniform proposals instead of freezing it.
 constexpr double kUniform = 1.0 / 32.0;
 
-// Activation: hard-swish, h(x) = x clamp(x / 6 + 1/2, 0, 1) (Howard et al. 2019, "Searching
-// for MobileNetV3").  Two or three VALU instructions and no transcendental, so the sweep's 96
-// activations per site cost little beside the MFMA layers; no approximation is involved, so
-// training and sweep evaluate the same FP32 expression.  h'(x) = 0 for x <= -3, 1 for x >= 3, x / 3 + 1/2
-// between.
-__device__ __forceinline__ float act(float x) { return x * fminf(fmaxf(fmaf(x, 1.0f / 6.0f, 0.5f), 0.0f), 1.0f); }
+// Activation: GELU in its tanh form (Hendrycks & Gimpel 2016),
+//   g(x) = x sigma(u),  u = c (x + 0.044715 x^3),  c = 2 sqrt(2 / pi),
+// which equals 0.5 x (1 + tanh(u / 2)).  Evaluated as x / (1 + 2^(-u log2 e)) with the
+// hardware exp2 and reciprocal: six VALU instructions, two of them transcendental, the same
+// FP32 expression in training and in the sweep.  g'(x) = sigma + x sigma (1 - sigma) u'(x).
+constexpr float kGeluC = 1.5957691216057308f;             // 2 sqrt(2 / pi)
+constexpr float kGeluK = 2.3022081983067571f;             // 2 sqrt(2 / pi) log2(e)
+__device__ __forceinline__ float gelu_sigma(float x) {
+  const float a = x * fmaf(x * x, 0.044715f * kGeluK, kGeluK);
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-a));
+}
+__device__ __forceinline__ float act(float x) { return x * gelu_sigma(x); }
 __device__ __forceinline__ float act_d(float x) {
-  return x <= -3.0f ? 0.0f : (x >= 3.0f ? 1.0f : fmaf(x, 1.0f / 3.0f, 0.5f));
+  const float sg = gelu_sigma(x);
+  return sg + x * sg * (1.0f - sg) * kGeluC * fmaf(x * x, 3.0f * 0.044715f, 1.0f);
 }
 
 // MLP forward (weights w in any address space).

@@ -4,7 +4,7 @@ it.  Sampling, record collection, forward/backward and Adam all run in libespins
 (esp_learn_*); this module only sequences the calls.
 
 The proposal for site i is a two-component von Mises-Fisher mixture whose concentration,
-weight and tilt an MLP (5 -> 32 -> 32 -> 32 -> 6, hard-swish) computes from the site's environment.
+weight and tilt an MLP (5 -> 32 -> 32 -> 32 -> 6, GELU) computes from the site's environment.
 Because it depends on the neighbours only, Metropolis-Hastings with the exact density ratio
 keeps detailed balance for any weights; training maximises the likelihood of equilibrium
 spins given their environment, which drives the proposal toward the heat-bath conditional.

@@ -46,14 +46,20 @@ def pack(p):
     return np.concatenate([np.asarray(p[name], np.float64).ravel() for name, _ in _SHAPES])
 
 
+GELU_C = 2.0 * math.sqrt(2.0 / math.pi)
+
+
 def act(x):
-    """Hard-swish, x clip(x / 6 + 1/2, 0, 1) (Howard et al. 2019, MobileNetV3)."""
-    return x * np.clip(x / 6.0 + 0.5, 0.0, 1.0)
+    """GELU, tanh form (Hendrycks & Gimpel 2016): 0.5 x (1 + tanh(c (x + 0.044715 x^3) / 2))
+    = x sigma(c (x + 0.044715 x^3)), c = 2 sqrt(2 / pi)."""
+    x = np.asarray(x, np.float64)
+    return 0.5 * x * (1.0 + np.tanh(0.5 * GELU_C * (x + 0.044715 * x ** 3)))
 
 
 def act_d(x):
-    """Its derivative, 0 at and below -3 and 1 at and above 3."""
-    return np.where(x <= -3.0, 0.0, np.where(x >= 3.0, 1.0, x / 3.0 + 0.5))
+    x = np.asarray(x, np.float64)
+    s = 0.5 * (1.0 + np.tanh(0.5 * GELU_C * (x + 0.044715 * x ** 3)))
+    return s + x * s * (1.0 - s) * GELU_C * (1.0 + 3.0 * 0.044715 * x * x)
 
 
 def forward(w, X):

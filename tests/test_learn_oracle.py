@@ -267,10 +267,11 @@ def test_bf16_round_matches_torch_bfloat16():
 def test_forward_sweep_net_hand_computed_case():
     """One path through the network with hand-chosen weights: x0 = 1 feeds hidden unit 0 of
     layer 1 with weight 1; layers 2 and 3 copy unit 0 with weight 1; output 0 reads unit 0 with
-    weight 1 and output 1 is the bias 0.1.  With h(x) = x clip(x / 6 + 1/2, 0, 1):
-      a1 = bf16(h(1))               = bf16(2/3)           = 0.66796875 (= 171/256)
-      a2 = bf16(h(0.66796875))      = bf16(0.40835571...) = 0.408203125 (= 209/512)
-      a3 = h(0.408203125)           (full precision)
+    weight 1 and output 1 is the bias 0.1.  With g the tanh-form GELU,
+    g(x) = 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))):
+      a1 = bf16(g(1))               = bf16(0.84119199...) = 0.83984375 (= 215/256)
+      a2 = bf16(g(0.83984375))      = bf16(0.67135989...) = 0.671875   (= 172/256)
+      a3 = g(0.671875)              (full precision)
       o0 = a3, o1 = bf16(0.1) = 0.10009765625, others 0"""
     p = {k: np.zeros(s) for k, s in LO._SHAPES}
     p["W1"][0, 0] = 1.0
@@ -280,11 +281,11 @@ def test_forward_sweep_net_hand_computed_case():
     p["b4"][1] = 0.1
     w = LO.pack(p)
     X = np.array([[1.0, 0.0, 0.0, 0.0, 0.0]])
-    h = lambda t: t * min(max(t / 6.0 + 0.5, 0.0), 1.0)
-    a1 = 171.0 / 256.0
+    h = lambda t: 0.5 * t * (1.0 + math.tanh(math.sqrt(2.0 / math.pi) * (t + 0.044715 * t ** 3)))
+    a1 = 215.0 / 256.0
     assert abs(h(1.0) - a1) < 2.0 ** -9        # within half a BF16 ulp at this magnitude
-    a2 = 209.0 / 512.0
-    assert abs(h(a1) - a2) < 2.0 ** -10
+    a2 = 172.0 / 256.0
+    assert abs(h(a1) - a2) < 2.0 ** -9
     a3 = h(a2)
     o = LO.forward_sweep_net(w, X)[0]
     assert o[0] == pytest.approx(a3, rel=1e-14)
@@ -295,3 +296,15 @@ def test_forward_sweep_net_hand_computed_case():
     want = h(h(h(1.0)))
     assert of[0, 0] == pytest.approx(want, rel=1e-14)
     assert abs(of[0, 0] - o[0]) > 1e-4
+
+
+def test_activation_is_torch_tanh_gelu():
+    """The network's activation and its derivative against an independent implementation:
+    torch's gelu(approximate='tanh') and its autograd gradient, in float64."""
+    import torch
+    x = torch.linspace(-12.0, 12.0, 4001, dtype=torch.float64, requires_grad=True)
+    y = torch.nn.functional.gelu(x, approximate="tanh")
+    (g,) = torch.autograd.grad(y.sum(), x)
+    xn = x.detach().numpy()
+    np.testing.assert_allclose(LO.act(xn), y.detach().numpy(), rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(LO.act_d(xn), g.numpy(), rtol=1e-12, atol=1e-14)
