@@ -294,6 +294,39 @@ __device__ __forceinline__ Mix make_mix(const float o[kOut], float kappa0, const
   return m;
 }
 
+// The mixture as the sweep uses it, without logarithms: selection weights ws_k = (1 - kUniform)
+// w_k, concentrations, e^{-2 kappa_k} and density prefactors c_k = ws_k kappa_k / (2 pi (1 -
+// e^{-2 kappa_k})), so that q(x) = sum_k c_k exp(kappa_k (mu_k . x - 1)) + kUniform / (4 pi).
+// Sampling and density use the same ws_k, kappa_k and mu_k (the log-domain Mix of training
+// differs from it only by rounding, which no sweep quantity depends on).
+struct MixS {
+  double ws0;       // selection weight of component 0 (component 1: 1 - kUniform - ws0)
+  double kappa[2], e2k[2], c[2];
+  V3 mu[2];
+};
+__device__ __forceinline__ MixS make_mix_sweep(const float o[kOut], float kappa0, const V3 &ghat, const V3 &ax) {
+  MixS m;
+  const V3 e = tilt_frame(ghat, ax);
+  const double w0 = (double)__builtin_amdgcn_rcpf(1.0f + __expf(o[4] - o[1]));   // softmax of the logits
+  const double ws[2] = {(1.0 - kUniform) * w0, (1.0 - kUniform) * (1.0 - w0)};
+  m.ws0 = ws[0];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const float sc = fminf(fmaxf(o[3 * k], -8.0f), 8.0f);
+    const double kap = (double)(kappa0 * expf(sc)) + 1e-6;
+    m.kappa[k] = kap;
+    // e^{-2 kappa} < 2^-54 for kappa > 19: expm1 would round to exactly -1 there
+    const double em1 = kap > 19.0 ? -1.0 : expm1(-2.0 * kap);
+    m.e2k[k] = 1.0 + em1;
+    m.c[k] = ws[k] * kap / (2.0 * kPi * -em1);
+    const float th = tanhf(o[3 * k + 2]);
+    float sn, cs;
+    sincospif(0.5f * th, &sn, &cs);
+    m.mu[k] = vnorm(V3{cs * ghat.x + sn * e.x, cs * ghat.y + sn * e.y, cs * ghat.z + sn * e.z});
+  }
+  return m;
+}
+
 // log vMF_k(x) = lnorm_k + kappa_k (mu_k . x - 1)
 __device__ __forceinline__ double log_vmf(const Mix &m, int k, const V3 &x) {
   return m.lnorm[k] + m.kappa[k] * (dot(m.mu[k], x) - 1.0);
@@ -464,14 +497,14 @@ __global__ __launch_bounds__(kBlock) ESP_LEARN_ATTR void k_phase_learned(KArgs a
   V3 ghat;
   features(G, QB, ax, Delta, beta, mag, x, kappa0, ghat);
   net_forward_mfma(w, load_net_a(wbf16), x, o);
-  const Mix mix = make_mix(o, kappa0, ghat, ax);
+  const MixS mix = make_mix_sweep(o, kappa0, ghat, ax);
   const int ir = a.dev2ref[d];
   const uint4 rn = philox10(make_uint4((uint32_t)ir, (uint32_t)sweep, (uint32_t)(sweep >> 32),
                                        a.stream_base + (uint32_t)r),
                             a.key0, a.key1);
   // component: uniform with probability kUniform, else network component 0 or 1
   const double uc = u01(rn.x);
-  const int comp = uc < kUniform ? 2 : (uc - kUniform < exp(mix.logw[0]) ? 0 : 1);
+  const int comp = uc < kUniform ? 2 : (uc - kUniform < mix.ws0 ? 0 : 1);
   // select by value (a dynamic index into the register-resident mixture would spill it)
   const bool one = comp == 1;
   const V3 xn = sample_vmf(comp == 2 ? 0.0 : (one ? mix.kappa[1] : mix.kappa[0]),
@@ -483,7 +516,7 @@ __global__ __launch_bounds__(kBlock) ESP_LEARN_ATTR void k_phase_learned(KArgs a
   const double dE = -(G.x * (sn.x - so.x) + G.y * (sn.y - so.y) + G.z * (sn.z - so.z)) + quad(Q, sn) - quad(Q, so);
   // Metropolis-Hastings ratio exp(-beta dE) q(xo) / q(xn) with q in the linear domain: the
   // uniform component keeps both densities >= kUniform / (4 pi) > 0
-  const double c0 = exp(mix.logw[0] + mix.lnorm[0]), c1 = exp(mix.logw[1] + mix.lnorm[1]);
+  const double c0 = mix.c[0], c1 = mix.c[1];
   const double cu = kUniform / (4.0 * kPi);
   const double qo = c0 * exp(mix.kappa[0] * (dot(mix.mu[0], xo) - 1.0)) +
                     c1 * exp(mix.kappa[1] * (dot(mix.mu[1], xo) - 1.0)) + cu;
