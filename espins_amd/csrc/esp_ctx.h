@@ -47,6 +47,7 @@ struct esp_ctx {
   int *d_slot_rep = nullptr, *d_rep_slot = nullptr;
   unsigned long long *d_rej = nullptr, *d_nswap = nullptr, *d_nmeas = nullptr;
   Clock *d_clock = nullptr;
+  unsigned *d_done = nullptr;    // blocks of k_reduce_fin done this step (the last one resets it)
   std::vector<double> tems;
   // host-fed staging
   double *d_prop = nullptr, *d_unif = nullptr;

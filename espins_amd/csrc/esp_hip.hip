@@ -255,6 +255,28 @@ int enqueue_finalize(esp_ctx *c, bool use_partials, bool measure, bool exchange,
   return ESP_OK;
 }
 
+// enqueue_reduce + enqueue_finalize(use_partials = true) in one launch (k_reduce_fin): with the
+// colour phases' per-wave sums, one block per 4096 entries of a replica, else the spin-sum blocks.
+int enqueue_reduce_finalize(esp_ctx *c, bool measure, bool exchange, int advance, uint64_t seed) {
+  const bool mp = c->mpart_valid;
+  const int ne = c->hm.ncol * c->mpart_nw;
+  const int nblk = mp ? std::max(1, std::min(c->red_blocks, (ne + 4095) / 4096)) : c->red_blocks;
+  FinArgs f{};
+  f.partial = c->d_partial;
+  f.nblk = nblk; f.N = c->hm.N; f.R = c->R;
+  f.e_base = c->d_ebase; f.e_run = c->d_erun; f.acc = c->d_acc; f.nmeas = c->d_nmeas;
+  f.slot_rep = c->d_slot_rep; f.rep_slot = c->d_rep_slot; f.beta_slot = c->d_beta;
+  f.nswap = c->d_nswap; f.clock = c->d_clock;
+  f.measure = measure; f.exchange = exchange; f.advance_sweeps = advance;
+  f.key0 = (uint32_t)seed; f.key1 = (uint32_t)(seed >> 32); f.stream_base = c->stream_base;
+  f.trace = c->d_trace; f.trace_cap = c->trace_cap;
+  hipLaunchKernelGGL(k_reduce_fin, dim3(nblk, c->R), dim3(kBlock), 0, c->stream, (const double *)c->d_sp,
+                     (const double *)c->d_w, c->hm.N, c->Ns, mp ? (const double *)c->d_mpart : nullptr, ne,
+                     c->d_partial, f, c->d_done);
+  HIPCHK(c, hipGetLastError());
+  return ESP_OK;
+}
+
 int set_clock(esp_ctx *c, uint64_t sweep, uint64_t step, int parity) {
   Clock h{sweep, step, parity, 0};
   HIPCHK(c, hipMemcpyAsync(c->d_clock, &h, sizeof(Clock), hipMemcpyHostToDevice, c->stream));
@@ -315,21 +337,15 @@ int enqueue_round(esp_ctx *c, int steps, int M, bool measure, bool exchange, int
   if (!measure) {
     rc = enqueue_phase_sweeps(c, mode, tilt, seed, steps * M, 0);
     if (rc) return rc;
-    if (exchange) {
-      rc = enqueue_reduce(c);
-      if (rc) return rc;
-      return enqueue_finalize(c, true, false, true, steps * M, seed);
-    }
+    if (exchange) return enqueue_reduce_finalize(c, false, true, steps * M, seed);
     return enqueue_finalize(c, false, false, false, steps * M, seed);
   }
   for (int s = 0; s < steps; ++s) {
     rc = enqueue_phase_sweeps(c, mode, tilt, seed, M, s * M);
     if (rc) return rc;
-    rc = enqueue_reduce(c);
-    if (rc) return rc;
-    if ((rc = esp::obs_enqueue(c))) return rc;
+    if ((rc = esp::obs_enqueue(c))) return rc;   // reads spins and slot labels only
     const bool last = s + 1 == steps;
-    rc = enqueue_finalize(c, true, true, exchange && last, last ? steps * M : 0, seed);
+    rc = enqueue_reduce_finalize(c, true, exchange && last, last ? steps * M : 0, seed);
     if (rc) return rc;
   }
   return ESP_OK;
@@ -361,7 +377,11 @@ int esp_create(esp_ctx **out, int device) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->lds_max = prop.sharedMemPerBlock;
-  if (hipMalloc((void **)&c->d_clock, sizeof(Clock)) != hipSuccess) {
+  if (hipMalloc((void **)&c->d_clock, sizeof(Clock)) != hipSuccess ||
+      hipMalloc((void **)&c->d_done, sizeof(unsigned)) != hipSuccess ||
+      hipMemset(c->d_done, 0, sizeof(unsigned)) != hipSuccess) {
+    dfree(c->d_clock);
+    dfree(c->d_done);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return ESP_ENOMEM;
@@ -382,6 +402,7 @@ int esp_destroy(esp_ctx *c) {
   esp::obs_free(c);
   esp::learn_free(c);
   dfree(c->d_clock);
+  dfree(c->d_done);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return ESP_OK;

@@ -727,6 +727,33 @@ __global__ __launch_bounds__(1024) void k_finalize(FinArgs f) {
   if (threadIdx.x == 0) advance_clock(f.clock, f.exchange, f.advance_sweeps);
 }
 
+// k_reduce followed by k_finalize in one launch: every block adds its partial sums
+// (reduce_body); the last block to finish -- counted on a device-scope counter that it resets
+// -- then runs finalize_body and advances the clock.  Each block's thread 0 releases its
+// partials with a device fence before counting; the last block's threads fence (acquire, which
+// also drops stale L1 lines) before reading them.
+__global__ __launch_bounds__(kBlock) void k_reduce_fin(const double *sp, const double *wpart, int N, int Ns,
+                                                       const double *mpart, int ne, double *partial, FinArgs f,
+                                                       unsigned *done) {
+  reduce_body(sp, wpart, N, Ns, f.nblk, partial, mpart, ne, blockIdx.y, blockIdx.x);
+  __shared__ int s_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    s_last = atomicAdd(done, 1u) == gridDim.x * gridDim.y - 1;
+  }
+  __syncthreads();
+  if (!s_last) return;   // uniform per block
+  __threadfence();
+  const int start = f.clock->parity;  // 1 or 2 (pt.F90:679,898)
+  const unsigned long long step = f.clock->step;
+  finalize_body(f, start, step);
+  if (threadIdx.x == 0) {
+    advance_clock(f.clock, f.exchange, f.advance_sweeps);
+    *done = 0u;
+  }
+}
+
 // Direct energy (energy_get, get_quant.F90:186-261): per-site terms, block partials.
 template <bool CMP, bool HB, bool HD>
 __global__ __launch_bounds__(kBlock) void k_energy(KArgs a, int nblk, double *partial) {
