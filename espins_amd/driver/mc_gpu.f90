@@ -32,7 +32,8 @@ program mc_gpu
   integer(c_int), allocatable :: smap(:)
   integer(c_int64_t) :: nmeas, seed, sweep
   integer :: w1, nsw_warm, nsw, steps_mc
-  integer(8) :: t0, t1, rate
+  integer(8) :: t0, t1, rate, t_start
+  real(dp) :: tm_j = 0, tm_b = 0, tm_d = 0, tm_mc = 0   ! stopwatches of the timing table
   real(dp) :: beta, C, chi, UE, UM, dE, dM, nrm, r1, r2
   type(mt_t) :: rng
   integer :: rank_seed
@@ -56,6 +57,7 @@ program mc_gpu
   k = index(seedname, '.mcin', back=.true.)
   if (k > 0) seedname = seedname(1:k-1)
   call system_clock(t0, rate)
+  t_start = t0
 
   call read_input(trim(seedname)//'.mcin', inp)
   open(newunit=out_u, file=trim(seedname)//'.mcout', status='replace', action='write')
@@ -83,18 +85,21 @@ program mc_gpu
   zB = 0
   zD = 0
   call system_clock(t1)
-  write(out_u, '(1x,a25,f11.3,a)') 'Time to get Jij matrix   ', real(t1 - t0, dp) / real(rate, dp), ' (sec)'
+  tm_j = real(t1 - t0, dp) / real(rate, dp)
+  write(out_u, '(1x,a25,f11.3,a)') 'Time to get Jij matrix   ', tm_j, ' (sec)'
   t0 = t1
   if (inp%have_B) then
     call build(inp%B, 1, zB, bc, bn, bv)
     call system_clock(t1)
-    write(out_u, '(1x,a25,f11.3,a)') 'Time to get Bij matrix   ', real(t1 - t0, dp) / real(rate, dp), ' (sec)'
+    tm_b = real(t1 - t0, dp) / real(rate, dp)
+    write(out_u, '(1x,a25,f11.3,a)') 'Time to get Bij matrix   ', tm_b, ' (sec)'
     t0 = t1
   end if
   if (inp%have_D) then
     call build(inp%D, 3, zD, dc, dn, dvv)
     call system_clock(t1)
-    write(out_u, '(1x,a25,f11.3,a)') 'Time to get Dij matrix   ', real(t1 - t0, dp) / real(rate, dp), ' (sec)'
+    tm_d = real(t1 - t0, dp) / real(rate, dp)
+    write(out_u, '(1x,a25,f11.3,a)') 'Time to get Dij matrix   ', tm_d, ' (sec)'
     t0 = t1
   end if
   if (inp%have_si) then
@@ -293,15 +298,41 @@ program mc_gpu
 
   call check(esp_destroy(ctx), 'esp_destroy')
   call system_clock(t1)
+  tm_mc = real(t1 - t0, dp) / real(rate, dp)
   if (inp%pt) then                  ! mc_prog.F90:170-185
-    write(out_u, '(/1x,a,f11.3,a)') 'Time for Parallel-Tempering montecarlo ', real(t1 - t0, dp) / real(rate, dp), ' (sec)'
+    write(out_u, '(/1x,a,f11.3,a)') 'Time for Parallel-Tempering montecarlo ', tm_mc, ' (sec)'
   else
-    write(out_u, '(/1x,a,f11.3,a)') 'Time for montecarlo ', real(t1 - t0, dp) / real(rate, dp), ' (sec)'
+    write(out_u, '(/1x,a,f11.3,a)') 'Time for montecarlo ', tm_mc, ' (sec)'
+  end if
+  ! closing lines and the stopwatch table (mc_prog.F90:199-208, io_print_timings io.F90:130-146)
+  write(out_u, '(/,1x,a25,f11.3,a)') 'Total Execution Time     ', real(t1 - t_start, dp) / real(rate, dp), ' (sec)'
+  write(out_u, '(/1x,a)') '*===================================================================================*'
+  write(out_u, '(1x,a)')  '|                                 TIMING INFORMATION                                |'
+  write(out_u, '(1x,a)')  '*===================================================================================*'
+  write(out_u, '(1x,a)')  '|    Tag                                                Ncalls      Time (s)        |'
+  write(out_u, '(1x,a)')  '|-----------------------------------------------------------------------------------|'
+  call timing_row('jij: get', tm_j)
+  if (inp%have_B) call timing_row('bij: get', tm_b)
+  if (inp%have_D) call timing_row('dij: get', tm_d)
+  call timing_row(merge('pt_main    ', 'mcarlo_main', inp%pt), tm_mc)
+  write(out_u, '(1x,a)')  '*-----------------------------------------------------------------------------------*'
+  if (inp%pt) then
+    write(out_u, '(/,1x,a)') 'All done: ptmc exiting'
+  else
+    write(out_u, '(/,1x,a)') 'All done: mcarlo exiting'
   end if
   close(out_u)
 224 format(3x,f9.4,1x,i14,1x,i14,5x,i14,7x,f10.8,4x,a)   ! pt.F90:1126
 
 contains
+
+  subroutine timing_row(tag, secs)
+    character(len=*), intent(in) :: tag
+    real(dp), intent(in) :: secs
+    character(len=48) :: t
+    t = tag
+    write(out_u, '(1x,"|",a48,":",i10,4x,f10.3,10x,"|")') t, 1, secs
+  end subroutine
 
   ! Spin glass (jij_spin_glass, jij.F90:506-556): the sigma of every slot of the J table from
   ! the sig= of the row that made it (a table of row numbers built over the same rows, zero-J

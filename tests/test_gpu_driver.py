@@ -42,6 +42,10 @@ def test_example01_plain_mc(tmp_path):
     # the published initial (ferro) energy, printed in the same E15.7 format
     # (examples/example01/reference/cubic.mcout:110)
     assert out.count("-0.6684205E+04  <-- ENERGY") == 20
+    # closing lines as mc_prog.F90:199-208 prints them
+    assert re.search(r"^ Total Execution Time\s+\d+\.\d{3} \(sec\)$", out, re.M)
+    assert re.search(r"^ \|mcarlo_main\s+:\s+1\s+\d+\.\d{3}\s+\|$", out, re.M)
+    assert out.rstrip().endswith("All done: mcarlo exiting")
     adds = [float(x) for x in re.findall(r"adding differences\s*=\s*(\S+)", out)]
     direct = [float(x) for x in re.findall(r"direct calculations\s*=\s*(\S+)", out)]
     assert len(adds) == 20 and np.allclose(adds, direct, rtol=1e-6)
@@ -84,6 +88,8 @@ def test_example02_parallel_tempering(tmp_path):
     perm = [int(x) for x in m[-1].split()]
     assert sorted(perm) == list(range(1, 33))
     assert "swaps" in out
+    assert re.search(r"^ \|pt_main\s+:\s+1\s+\d+\.\d{3}\s+\|$", out, re.M)
+    assert out.rstrip().endswith("All done: ptmc exiting")
     # per temperature slot: running energy = direct energy, and the direct energy is the
     # oracle's energy of the configuration written for that slot (3F12.6 spins, E15.7 print)
     adds = [float(x) for x in re.findall(r"adding differences\s*=\s*(\S+)\s+<-- ENERGY3", out)]
