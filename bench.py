@@ -519,6 +519,34 @@ def measured_traffic(config, kernel):
 STEP_DEFAULTS = {"batch_c3": (200, 20), "learned_fcc32": (2000, 2000), "copy_probe": (3, 0)}
 
 
+def enable_observables(eng, m, config, which="all"):
+    """The optional observables of the reference with their default sizes: order parameter
+    'op' (axes along z), staggered moment 'sm' (alternating signs over the basis), 'pm' P(m),
+    'sc' spin correlations over 2 shells, and 'sf' S(q) on the default 51 x 51 mesh spanned by
+    b1 and b3 every 100 measurements (parameters.F90:440-592)."""
+    sel = {"op", "sm", "pm", "sc", "sf"} if which == "all" else set(which.split(","))
+    import numpy as np
+    from espins_amd import lattice as Lt
+    kind = next(k for k in ("fcc", "bcc", "sc") if k in config)
+    L = int("".join(ch for ch in config.split(kind)[1] if ch.isdigit()) or 0)
+    nb = m.nb
+    if "op" in sel:
+        eng.set_orderparam(np.tile([0.0, 0.0, 1.0], (nb, 1)))
+    if "sm" in sel:
+        eng.set_staggered(np.array([1.0 if a % 2 == 0 else -1.0 for a in range(nb)]))
+    if "pm" in sel:
+        eng.set_pm(True)
+    if "sc" in sel:
+        cnt, nbr = Lt.shell_tables(kind, L, 2)
+        eng.set_spincorr(np.zeros(m.N, np.int32), cnt, nbr)
+    if "sf" in sel:
+        b = 2.0 * np.pi * np.eye(3)                # reciprocal vectors of the cubic cell
+        ij = np.array([(i, j) for i in range(51) for j in range(51)], float) / 50.0
+        q = ij[:, :1] * b[0] + ij[:, 1:] * b[2]
+        eng.set_sfactor(q, Lt.site_positions(kind, L), np.array([1.0, 0.0, 0.0]), every=100)
+    log(f"observables on: {sorted(sel)} (S(q): 2601 q-points every 100 measurements)")
+
+
 def run_single(args, config, steps, warmup, with_cpu=True, emit=True):
     """One single-context workload (plain MC or PT of one replica ladder); returns the JSON
     line as a dict (and prints it when emit)."""
@@ -541,6 +569,8 @@ def run_single(args, config, steps, warmup, with_cpu=True, emit=True):
     eng.set_path(args.path)
     eng.set_tps(args.tps)
     eng.set_replicas(tems, Lt.random_spins(m.N, R, seed, m.mag))
+    if getattr(args, "observables", None):
+        enable_observables(eng, m, config, args.observables)
     info = eng.model_info()
     log(f"{config} model: {info}")
     pt = R > 1 and config.startswith("pt")
@@ -602,7 +632,8 @@ def run_single(args, config, steps, warmup, with_cpu=True, emit=True):
         "data": "synthetic",
         "config": {"workload": desc, "N": m.N, "replicas": R, "sweeps_per_step": 1,
                    "exchange_every": 1 if pt else 0, "mode": "random", "colours": info["ncol"],
-                   "kernel_path": "lds" if info["lds"] else "global", "graph": graph},
+                   "kernel_path": "lds" if info["lds"] else "global", "graph": graph,
+                   "observables": getattr(args, "observables", None)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": (f"profiles/pmc_{kernel}_{config}.txt (FETCH_SIZE + WRITE_SIZE, "
@@ -636,6 +667,9 @@ def main():
     ap.add_argument("--path", type=int, default=0, help="0 auto, 1 global-memory phases, 2 LDS")
     ap.add_argument("--tps", type=int, default=0, help="lanes per site (0 automatic)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--observables", default=None,
+                    help="single-ladder configs: optional observables measured on the device, 'all' or a "
+                         "comma list of op, sm, pm, sc, sf")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
