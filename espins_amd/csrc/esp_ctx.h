@@ -86,10 +86,12 @@ struct esp_ctx {
     int nfreq = 0;
     double m_min = 0.0, m_max = 0.0;
     unsigned long long *d_pm = nullptr;
-    // spin correlations per shell: tables by reference site
-    int nshell = 0, nsp = 0, zs = 0;
-    int *d_species = nullptr, *d_scnt = nullptr, *d_snbr = nullptr;
-    double *d_snorm = nullptr;     // [nshell][nsp][nsp]
+    // spin correlations per shell: bond tables by device site
+    int nshell = 0, nsp = 0, zs = 0, srows = 0;
+    int *d_skind = nullptr;        // [N] kind of each site
+    int *d_soff = nullptr;         // [nshell+1] first table row of each shell
+    unsigned *d_sbond = nullptr;   // [srows][N] partner | kind of partner << 30
+    double *d_sw = nullptr;        // [1 or 2][srows][N] weights (sum; or forward, reverse)
     double *d_sc = nullptr;        // [R][3][nshell][nsp][nsp]: sum, |sum|, sum |.|
     double *d_scpart = nullptr;    // [R][site blocks][nshell][2][nsp][nsp]
     int sblocks = 0;
@@ -97,7 +99,10 @@ struct esp_ctx {
     int nq = 0, sf_every = 1;
     double polar[3] = {0, 0, 0};
     int have_polar = 0;
-    double *d_q = nullptr, *d_pos = nullptr;   // [nq][3], [N][3] by reference site
+    double *d_q = nullptr, *d_pos = nullptr;   // [nq][3], [N][3] by device site
+    double *d_sfspin = nullptr;    // [replica groups][N][8][3] spins of an S(q) evaluation
+    double *d_sfpart = nullptr;    // [site chunks][R][6][nq] partial F0
+    int sf_chunk = 0, sf_nchunk = 0;
     double *d_sf = nullptr;        // [R][3][nq]
     unsigned long long *d_sfn = nullptr;      // structure-factor measurements
     // per-measurement |OP| and |SM| by slot while the trace is on: [trace_cap][R][2]

@@ -9,12 +9,14 @@
 // The reference evaluates these on the host copy of one configuration per rank and
 // measurement; here one measurement of all replicas is a few launches over the spins in HBM:
 // k_obs_cells (one thread per cell: the cell sums, block partials of OP and SM, the P(m)
-// counts through a histogram in LDS), k_obs_spincorr (one thread per site), k_obs_sfactor
-// (one thread per q-point and four replicas: the phase of a site is computed once for all
-// four) and k_obs_final (per replica: fixed-order sums of the block partials, accumulation
-// into the replica's current slot).  Histogram counts are integers, so they do not depend on
-// the order of the atomic additions; every floating-point sum has a fixed order.
+// counts through a histogram in LDS), k_obs_spincorr (one thread per site and several
+// replicas, device order), k_sf_pack / k_sf_main / k_sf_fin (S(q) as a phase-matrix times
+// spin-matrix product: one phase per q-point, site and 8 replicas) and k_obs_final (per
+// replica: fixed-order sums of the block partials, accumulation into the replica's slot).
+// Histogram counts are integers, so they do not depend on the order of the atomic additions;
+// every floating-point sum has a fixed order.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -26,8 +28,12 @@ using namespace espk;
 
 constexpr int kObsBlock = 256;
 constexpr int kMaxSpecies = 4;        // kinds in the spin-correlation tables
-constexpr int kSfBlock = 128;         // q-points per structure-factor block
-constexpr int kSfRep = 4;             // replicas per structure-factor block
+constexpr int kBondKindShift = 30;    // spin-correlation bond: partner | kind of partner << 30
+constexpr unsigned kBondIdx = (1u << kBondKindShift) - 1u;
+constexpr int kSfQ = 64;              // q-points per structure-factor wave (one per lane)
+constexpr int kSfRep = 8;             // replicas per structure-factor wave
+constexpr int kSfWaves = 4096;        // structure-factor waves aimed at (site chunks fill up to it)
+constexpr int kSfMaxChunks = 64;
 constexpr int kPmLdsBins = 12288;     // P(m) bins counted in LDS (48 KiB); more use global atomics
 
 struct ObsArgs {
@@ -42,13 +48,16 @@ struct ObsArgs {
   unsigned long long *pm;
   int nfreq, ndim, pm_lds;
   double m_min, m_max;
-  const int *species, *scnt, *snbr;
-  const double *snorm;
-  int nshell, nsp, zs, sblocks;
+  const int *skind, *soff;
+  const unsigned *sbond;
+  const double *sw;
+  int nshell, nsp, zs, sblocks, srows;
   double *scpart, *sc;
-  const double *q, *pos;
+  const double *q, *pos;               // pos by device site
   double polar[3];
   int have_polar, nq, every;
+  double *sfspin, *sfpart;
+  int sf_chunk, sf_nchunk;
   double *sf;
   unsigned long long *sfn;
   double *acc;
@@ -125,73 +134,118 @@ __global__ __launch_bounds__(kObsBlock) void k_obs_cells(ObsArgs o) {
   }
 }
 
-// One thread per site i (reference order): per shell, the sums of S_i . S_j / (n m_i m_j) and
-// of |S_i . S_j| / (n m_i m_j) over the shell's neighbours j of each kind, as block partials
-// [shell][2][kind i][kind j].
+// One thread per site (device order) and REP replicas: per shell, the sums of
+// S_i . S_j / (n m_i m_j) and of |S_i . S_j| / (n m_i m_j) over the shell's bonds, by kind
+// pair, as block partials [shell][2][kind i][kind j].  A table entry of site i names a partner
+// j (device index, its kind in the top two bits) and the weights of i -> j and, when the
+// shell is symmetric, of j -> i (both directions of the bond from one gather); shell s has
+// rows soff[s] .. soff[s+1]-1.  A bond costs coalesced table loads shared by the REP
+// replicas and one spin gather per replica.  (Keeping a replica group on one
+// XCD, as the sweep does, measured slower here: 468 against 404 us on FCC 32^3 x 64.)
+template <int NS, int REP>
 __global__ __launch_bounds__(kObsBlock) void k_obs_spincorr(ObsArgs o) {
-  __shared__ double red[2 * kMaxSpecies * kMaxSpecies * (kObsBlock / 64)];
-  const int r = blockIdx.y;
-  const int i = blockIdx.x * kObsBlock + threadIdx.x;
-  const bool on = i < o.N;
-  const int ns = o.nsp;
-  int ki = 0;
-  V3 si = {0.0, 0.0, 0.0};
-  double mi = 1.0;
-  if (on) {
-    const int d = gldi(o.ref2dev, i);
-    ki = gldi(o.species, i);
-    si = spin_at(o, r, d);
-    mi = gldi(o.mag, d);
+  constexpr int NV = 2 * NS * NS * REP;
+  __shared__ double red[NV * (kObsBlock / 64)];
+  const int b = blockIdx.x;
+  const int r0 = blockIdx.y * REP;
+  const int d0 = b * kObsBlock + threadIdx.x;
+  const bool on = d0 < o.N;
+  const int d = on ? d0 : 0;
+  const int ki = NS > 1 ? gldi(o.skind, d) : 0;
+  V3 si[REP];
+  int ru[REP];
+#pragma unroll
+  for (int u = 0; u < REP; ++u) {
+    ru[u] = min(r0 + u, o.R - 1);              // replicas past R read a valid one, weighted 0
+    const V3 s = spin_at(o, ru[u], d);
+    const bool use = on && r0 + u < o.R;
+    si[u] = {use ? s.x : 0.0, use ? s.y : 0.0, use ? s.z : 0.0};
   }
   for (int s = 0; s < o.nshell; ++s) {
-    double v[2 * kMaxSpecies * kMaxSpecies];
+    double v[REP][2][NS], vr[REP][2][NS];     // [sum, sum of |.|][kind j]: (i, j) and (j, i)
 #pragma unroll
-    for (int k = 0; k < 2 * kMaxSpecies * kMaxSpecies; ++k) v[k] = 0.0;
-    if (on) {
-      const int n = gldi(o.scnt, (size_t)s * o.N + i);
-      for (int l = 0; l < n; ++l) {
-        const int j = gldi(o.snbr, ((size_t)s * o.zs + l) * o.N + i);
-        const int dj = gldi(o.ref2dev, j);
-        const int kj = gldi(o.species, j);
-        const V3 sj = spin_at(o, r, dj);
-        const double den = gldi(o.snorm, ((size_t)s * ns + ki) * ns + kj) * mi * gldi(o.mag, dj);
-        const double dd = si.x * sj.x + si.y * sj.y + si.z * sj.z;
-        const double t = dd / den, ta = fabs(dd) / den;
+    for (int u = 0; u < REP; ++u)
 #pragma unroll
-        for (int x = 0; x < kMaxSpecies; ++x)   // static register indices
+      for (int y = 0; y < NS; ++y) { v[u][0][y] = v[u][1][y] = vr[u][0][y] = vr[u][1][y] = 0.0; }
+    const int l1 = gldi(o.soff, s + 1);
+    for (int l = gldi(o.soff, s); l < l1; ++l) {
+      const unsigned e = gldi(o.sbond, (size_t)l * o.N + d);
+      const double w = gldi(o.sw, (size_t)l * o.N + d);
+      const double wr = NS > 1 ? gldi(o.sw, ((size_t)o.srows + l) * o.N + d) : 0.0;
+      const int dj = (int)(e & kBondIdx);
+      const int kj = NS > 1 ? (int)(e >> kBondKindShift) : 0;
 #pragma unroll
-          for (int y = 0; y < kMaxSpecies; ++y)
-            if (x == ki && y == kj) {
-              v[x * kMaxSpecies + y] += t;
-              v[kMaxSpecies * kMaxSpecies + x * kMaxSpecies + y] += ta;
-            }
+      for (int u = 0; u < REP; ++u) {
+        const V3 sj = spin_at(o, ru[u], dj);
+        const double x = si[u].x * sj.x + si[u].y * sj.y + si[u].z * sj.z;
+        const double t = x * w, ta = fabs(x) * w;
+#pragma unroll
+        for (int y = 0; y < NS; ++y)
+          if (y == kj) {
+            v[u][0][y] += t; v[u][1][y] += ta;
+            if (NS > 1) { vr[u][0][y] += x * wr; vr[u][1][y] += fabs(x) * wr; }
+          }
       }
     }
-    block_sum<2 * kMaxSpecies * kMaxSpecies>(v, red);
-    if (threadIdx.x == 0) {
-      double *p = o.scpart + (((size_t)r * o.sblocks + blockIdx.x) * o.nshell + s) * 2 * ns * ns;
+    double f[NV];
 #pragma unroll
-      for (int w = 0; w < 2; ++w)
+    for (int u = 0; u < REP; ++u)
 #pragma unroll
-        for (int a = 0; a < kMaxSpecies; ++a)
+      for (int t = 0; t < 2; ++t)
 #pragma unroll
-          for (int b = 0; b < kMaxSpecies; ++b)
-            if (a < ns && b < ns)
-              gsti(p, (size_t)(w * ns + a) * ns + b, v[w * kMaxSpecies * kMaxSpecies + a * kMaxSpecies + b]);
-    }
+        for (int x = 0; x < NS; ++x)
+#pragma unroll
+          for (int y = 0; y < NS; ++y)
+            f[((u * 2 + t) * NS + x) * NS + y] = (x == ki ? v[u][t][y] : 0.0) + (y == ki ? vr[u][t][x] : 0.0);
+    block_sum<NV>(f, red);
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int u = 0; u < REP; ++u)
+        if (r0 + u < o.R) {
+          double *p = o.scpart + (((size_t)(r0 + u) * o.sblocks + b) * o.nshell + s) * 2 * NS * NS;
+#pragma unroll
+          for (int k = 0; k < 2 * NS * NS; ++k) gsti(p, k, f[u * 2 * NS * NS + k]);
+        }
   }
 }
 
-// One thread per q-point, kSfRep replicas per block: F0(q) = sum_j S_j exp(i q.r_j), then the
-// component of F0 perpendicular to q, F, and NSF = Re(polar . F) / |polar|; adds |F|^2,
-// |F|^2 - NSF^2 and NSF^2 to the slot's sums (one writer per (slot, q)).  q = 0 adds nothing.
-__global__ __launch_bounds__(kSfBlock) void k_obs_sfactor(ObsArgs o) {
-  __shared__ double s_pos[kSfBlock][3];
-  __shared__ double s_sp[kSfRep][kSfBlock][3];
-  const unsigned long long cnt = gldi(o.nmeas, 0) + 1ull;
-  if (cnt % (unsigned long long)o.every) return;       // uniform over the grid
-  const int iq = blockIdx.x * kSfBlock + threadIdx.x;
-  const int r0 = blockIdx.y * kSfRep;
+// The structure factor F0(q) = sum_j S_j exp(i q.r_j) of every replica is a product of the
+// phase matrix [q][site] with the spin matrix [site][replica, component]; a phase is computed
+// once per (q, site) and kSfRep replicas.  Three launches, each a no-op unless this
+// measurement is one that evaluates S(q) (uniform over the grid):
+//   k_sf_pack   the spins, site-major in groups of kSfRep replicas: [group][d][kSfRep][3];
+//   k_sf_main   one wave per (64 q-points, replica group, site chunk): lane = q-point, the
+//               site's position and its kSfRep x 3 spins are wave-uniform loads (SGPRs),
+//               one sincos per site serves kSfRep x 6 FMAs (the wave is bound by its FP64
+//               instructions; 16 replicas per wave (248 VGPRs) and phases shared by four
+//               waves through LDS both measured slower);
+//               partial sums [chunk][replica][6][q];
+//   k_sf_fin    one thread per (q, replica): the chunk partials in order, then F, the
+//               component of F0 perpendicular to q, and NSF = Re(polar . F) / |polar|; adds
+//               |F|^2, |F|^2 - NSF^2 and NSF^2 to the slot's sums.  q = 0 adds nothing.
+__device__ __forceinline__ bool sf_due(const ObsArgs &o) {
+  return (gldi(o.nmeas, 0) + 1ull) % (unsigned long long)o.every == 0ull;
+}
+
+__global__ __launch_bounds__(kObsBlock) void k_sf_pack(ObsArgs o) {
+  if (!sf_due(o)) return;
+  const int d = blockIdx.x * kObsBlock + threadIdx.x;
+  if (d >= o.N) return;
+  const int g = blockIdx.y;
+  double *out = o.sfspin + ((size_t)g * o.N + d) * (kSfRep * 3);
+#pragma unroll
+  for (int u = 0; u < kSfRep; ++u) {
+    const int r = g * kSfRep + u;
+    const V3 s = r < o.R ? spin_at(o, r, d) : V3{0.0, 0.0, 0.0};
+    gsti(out, 3 * u, s.x); gsti(out, 3 * u + 1, s.y); gsti(out, 3 * u + 2, s.z);
+  }
+}
+
+__global__ __launch_bounds__(kSfQ) void k_sf_main(ObsArgs o) {
+  if (!sf_due(o)) return;
+  const int iq = blockIdx.x * kSfQ + threadIdx.x;
+  const int g = blockIdx.y, ch = blockIdx.z;
+  const int d0 = ch * o.sf_chunk, d1 = min(o.N, d0 + o.sf_chunk);
   double qx = 0.0, qy = 0.0, qz = 0.0;
   if (iq < o.nq) { qx = gldi(o.q, 3 * iq); qy = gldi(o.q, 3 * iq + 1); qz = gldi(o.q, 3 * iq + 2); }
   double fr[kSfRep][3], fi[kSfRep][3];
@@ -199,62 +253,82 @@ __global__ __launch_bounds__(kSfBlock) void k_obs_sfactor(ObsArgs o) {
   for (int u = 0; u < kSfRep; ++u)
 #pragma unroll
     for (int k = 0; k < 3; ++k) { fr[u][k] = 0.0; fi[u][k] = 0.0; }
-  for (int j0 = 0; j0 < o.N; j0 += kSfBlock) {
-    const int j = j0 + threadIdx.x;
-    __syncthreads();
-    if (j < o.N) {
-      for (int k = 0; k < 3; ++k) s_pos[threadIdx.x][k] = gldi(o.pos, 3 * (size_t)j + k);
-      const int d = gldi(o.ref2dev, j);
-      for (int u = 0; u < kSfRep; ++u) {
-        const V3 s = r0 + u < o.R ? spin_at(o, r0 + u, d) : V3{0.0, 0.0, 0.0};
-        s_sp[u][threadIdx.x][0] = s.x; s_sp[u][threadIdx.x][1] = s.y; s_sp[u][threadIdx.x][2] = s.z;
+  const double *spg = o.sfspin + (size_t)g * o.N * (kSfRep * 3);
+  for (int d = d0; d < d1; ++d) {
+    const double *p = o.pos + 3 * (size_t)d;
+    double sn, cs;
+    sincos(qx * p[0] + qy * p[1] + qz * p[2], &sn, &cs);
+    const double *sv = spg + (size_t)d * (kSfRep * 3);
+#pragma unroll
+    for (int u = 0; u < kSfRep; ++u)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        fr[u][k] += sv[3 * u + k] * cs;
+        fi[u][k] += sv[3 * u + k] * sn;
       }
-    }
-    __syncthreads();
-    const int nt = min(kSfBlock, o.N - j0);
-    for (int t = 0; t < nt; ++t) {
-      double sn, cs;
-      sincos(qx * s_pos[t][0] + qy * s_pos[t][1] + qz * s_pos[t][2], &sn, &cs);
+  }
+  if (iq >= o.nq) return;
 #pragma unroll
-      for (int u = 0; u < kSfRep; ++u)
+  for (int u = 0; u < kSfRep; ++u) {
+    const int r = g * kSfRep + u;
+    if (r >= o.R) break;
+    double *out = o.sfpart + ((size_t)ch * o.R + r) * 6 * o.nq + iq;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          fr[u][k] += s_sp[u][t][k] * cs;
-          fi[u][k] += s_sp[u][t][k] * sn;
-        }
+    for (int k = 0; k < 3; ++k) {
+      gsti(out, (size_t)k * o.nq, fr[u][k]);
+      gsti(out, (size_t)(3 + k) * o.nq, fi[u][k]);
     }
   }
+}
+
+__global__ __launch_bounds__(kObsBlock) void k_sf_fin(ObsArgs o) {
+  if (!sf_due(o)) return;
+  const int iq = blockIdx.x * kObsBlock + threadIdx.x;
+  const int r = blockIdx.y;
+  if (iq >= o.nq) return;
+  const double qx = gldi(o.q, 3 * iq), qy = gldi(o.q, 3 * iq + 1), qz = gldi(o.q, 3 * iq + 2);
   const double q2 = qx * qx + qy * qy + qz * qz;
-  if (iq >= o.nq || q2 == 0.0) return;
+  if (q2 == 0.0) return;
+  double fr[3] = {0.0, 0.0, 0.0}, fi[3] = {0.0, 0.0, 0.0};
+  for (int ch = 0; ch < o.sf_nchunk; ++ch) {
+    const double *p = o.sfpart + ((size_t)ch * o.R + r) * 6 * o.nq + iq;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      fr[k] += gldi(p, (size_t)k * o.nq);
+      fi[k] += gldi(p, (size_t)(3 + k) * o.nq);
+    }
+  }
   const double pn = o.have_polar ? sqrt(o.polar[0] * o.polar[0] + o.polar[1] * o.polar[1] +
                                         o.polar[2] * o.polar[2]) : 1.0;
-  for (int u = 0; u < kSfRep && r0 + u < o.R; ++u) {
-    // perpendicular part: F - (F . q / q^2) q (real and imaginary parts separately)
-    const double ar = (fr[u][0] * qx + fr[u][1] * qy + fr[u][2] * qz) / q2;
-    const double ai = (fi[u][0] * qx + fi[u][1] * qy + fi[u][2] * qz) / q2;
-    const double pr[3] = {fr[u][0] - ar * qx, fr[u][1] - ar * qy, fr[u][2] - ar * qz};
-    const double pi[3] = {fi[u][0] - ai * qx, fi[u][1] - ai * qy, fi[u][2] - ai * qz};
-    const double f2 = pr[0] * pr[0] + pr[1] * pr[1] + pr[2] * pr[2] + pi[0] * pi[0] + pi[1] * pi[1] +
-                      pi[2] * pi[2];
-    double nsf = 0.0;
-    if (o.have_polar) nsf = (o.polar[0] * pr[0] + o.polar[1] * pr[1] + o.polar[2] * pr[2]) / pn;
-    const int slot = gldi(o.rep_slot, r0 + u);
-    double *acc = o.sf + (size_t)slot * 3 * o.nq;
-    gsti(acc, iq, gldi(acc, iq) + f2);
-    gsti(acc, (size_t)o.nq + iq, gldi(acc, (size_t)o.nq + iq) + (f2 - nsf * nsf));
-    gsti(acc, 2 * (size_t)o.nq + iq, gldi(acc, 2 * (size_t)o.nq + iq) + nsf * nsf);
-  }
+  // perpendicular part: F - (F . q / q^2) q (real and imaginary parts separately)
+  const double ar = (fr[0] * qx + fr[1] * qy + fr[2] * qz) / q2;
+  const double ai = (fi[0] * qx + fi[1] * qy + fi[2] * qz) / q2;
+  const double pr[3] = {fr[0] - ar * qx, fr[1] - ar * qy, fr[2] - ar * qz};
+  const double pi[3] = {fi[0] - ai * qx, fi[1] - ai * qy, fi[2] - ai * qz};
+  const double f2 = pr[0] * pr[0] + pr[1] * pr[1] + pr[2] * pr[2] + pi[0] * pi[0] + pi[1] * pi[1] +
+                    pi[2] * pi[2];
+  double nsf = 0.0;
+  if (o.have_polar) nsf = (o.polar[0] * pr[0] + o.polar[1] * pr[1] + o.polar[2] * pr[2]) / pn;
+  const int slot = gldi(o.rep_slot, r);
+  double *acc = o.sf + (size_t)slot * 3 * o.nq;
+  gsti(acc, iq, gldi(acc, iq) + f2);
+  gsti(acc, (size_t)o.nq + iq, gldi(acc, (size_t)o.nq + iq) + (f2 - nsf * nsf));
+  gsti(acc, 2 * (size_t)o.nq + iq, gldi(acc, 2 * (size_t)o.nq + iq) + nsf * nsf);
 }
 
 // One block per replica: fixed-order sums of the block partials, accumulation by slot.
 __global__ __launch_bounds__(kObsBlock) void k_obs_final(ObsArgs o) {
+  __shared__ double red[4 * (kObsBlock / 64)];
   const int r = blockIdx.x;
   const int slot = gldi(o.rep_slot, r);
   const unsigned long long nm = gldi(o.nmeas, 0);
-  if ((o.flags & (ESP_OBS_ORDER | ESP_OBS_STAGGERED)) && threadIdx.x == 0) {
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int b = 0; b < o.cblocks; ++b)
+  double v[4] = {0.0, 0.0, 0.0, 0.0};    // OP, SM x, y, z: strided over the cell blocks
+  if (o.flags & (ESP_OBS_ORDER | ESP_OBS_STAGGERED)) {   // uniform: block_sum needs all threads
+    for (int b = threadIdx.x; b < o.cblocks; b += kObsBlock)
       for (int k = 0; k < 4; ++k) v[k] += gldi(o.part, ((size_t)r * o.cblocks + b) * 4 + k);
+    block_sum<4>(v, red);
+  }
+  if ((o.flags & (ESP_OBS_ORDER | ESP_OBS_STAGGERED)) && threadIdx.x == 0) {
     const double n = (double)o.N;
     const double op = v[0] / n;
     const double sx = v[1] / n, sy = v[2] / n, sz = v[3] / n;
@@ -276,21 +350,24 @@ __global__ __launch_bounds__(kObsBlock) void k_obs_final(ObsArgs o) {
       gsti(t, 1, sqrt(sm2));
     }
   }
-  if (o.flags & ESP_OBS_SPINCORR) {
+  if (o.flags & ESP_OBS_SPINCORR) {      // uniform over the block: block_sum needs all threads
     const int ns = o.nsp, kk = ns * ns;
-    for (int e = threadIdx.x; e < o.nshell * kk; e += kObsBlock) {
+    for (int e = 0; e < o.nshell * kk; ++e) {
       const int s = e / kk, ab = e % kk;
-      double dot = 0.0, ab_sum = 0.0;
-      for (int b = 0; b < o.sblocks; ++b) {
+      double w[2] = {0.0, 0.0};               // dot, sum of |dot|: strided over the site blocks
+      for (int b = threadIdx.x; b < o.sblocks; b += kObsBlock) {
         const double *p = o.scpart + (((size_t)r * o.sblocks + b) * o.nshell + s) * 2 * kk;
-        dot += gldi(p, ab);
-        ab_sum += gldi(p, (size_t)kk + ab);
+        w[0] += gldi(p, ab);
+        w[1] += gldi(p, (size_t)kk + ab);
       }
-      double *a = o.sc + (size_t)slot * 3 * o.nshell * kk;
-      const size_t q = (size_t)s * kk + ab;
-      gsti(a, q, gldi(a, q) + dot);
-      gsti(a, (size_t)o.nshell * kk + q, gldi(a, (size_t)o.nshell * kk + q) + fabs(dot));
-      gsti(a, 2 * (size_t)o.nshell * kk + q, gldi(a, 2 * (size_t)o.nshell * kk + q) + ab_sum);
+      block_sum<2>(w, red);
+      if (threadIdx.x == 0) {
+        double *a = o.sc + (size_t)slot * 3 * o.nshell * kk;
+        const size_t q = (size_t)s * kk + ab;
+        gsti(a, q, gldi(a, q) + w[0]);
+        gsti(a, (size_t)o.nshell * kk + q, gldi(a, (size_t)o.nshell * kk + q) + fabs(w[0]));
+        gsti(a, 2 * (size_t)o.nshell * kk + q, gldi(a, 2 * (size_t)o.nshell * kk + q) + w[1]);
+      }
     }
   }
   if ((o.flags & ESP_OBS_SFACTOR) && r == 0 && threadIdx.x == 0 && (nm + 1ull) % (unsigned long long)o.every == 0)
@@ -311,12 +388,13 @@ ObsArgs obs_args(esp_ctx *c) {
   o.axes = b.d_axes; o.coeff = b.d_coeff; o.part = b.d_part; o.cblocks = b.cblocks;
   o.pm = b.d_pm; o.nfreq = b.nfreq; o.ndim = ndim_pm(c); o.m_min = b.m_min; o.m_max = b.m_max;
   o.pm_lds = o.ndim * o.nfreq <= kPmLdsBins;
-  o.species = b.d_species; o.scnt = b.d_scnt; o.snbr = b.d_snbr; o.snorm = b.d_snorm;
+  o.skind = b.d_skind; o.soff = b.d_soff; o.sbond = b.d_sbond; o.sw = b.d_sw; o.srows = b.srows;
   o.nshell = b.nshell; o.nsp = b.nsp; o.zs = b.zs; o.sblocks = b.sblocks;
   o.scpart = b.d_scpart; o.sc = b.d_sc;
   o.q = b.d_q; o.pos = b.d_pos;
   for (int k = 0; k < 3; ++k) o.polar[k] = b.polar[k];
   o.have_polar = b.have_polar; o.nq = b.nq; o.every = b.sf_every < 1 ? 1 : b.sf_every;
+  o.sfspin = b.d_sfspin; o.sfpart = b.d_sfpart; o.sf_chunk = b.sf_chunk; o.sf_nchunk = b.sf_nchunk;
   o.sf = b.d_sf; o.sfn = b.d_sfn;
   o.acc = b.d_acc; o.trace = b.d_trace; o.trace_cap = c->trace_cap;
   return o;
@@ -342,15 +420,16 @@ namespace esp {
 void obs_free(esp_ctx *c) {
   auto &b = c->obs;
   dfree(b.d_axes); dfree(b.d_coeff); dfree(b.d_acc); dfree(b.d_part); dfree(b.d_pm);
-  dfree(b.d_species); dfree(b.d_scnt); dfree(b.d_snbr); dfree(b.d_snorm); dfree(b.d_sc);
+  dfree(b.d_skind); dfree(b.d_soff); dfree(b.d_sbond); dfree(b.d_sw); dfree(b.d_sc);
   dfree(b.d_scpart); dfree(b.d_q); dfree(b.d_pos); dfree(b.d_sf); dfree(b.d_sfn); dfree(b.d_trace);
+  dfree(b.d_sfspin); dfree(b.d_sfpart);
   b = esp_ctx::Obs{};
 }
 
 int obs_resize(esp_ctx *c) {
   auto &b = c->obs;
   dfree(b.d_acc); dfree(b.d_part); dfree(b.d_pm); dfree(b.d_sc); dfree(b.d_scpart);
-  dfree(b.d_sf); dfree(b.d_sfn); dfree(b.d_trace);
+  dfree(b.d_sf); dfree(b.d_sfn); dfree(b.d_trace); dfree(b.d_sfspin); dfree(b.d_sfpart);
   if (!b.flags || c->R == 0) return ESP_OK;
   const int R = c->R, N = c->hm.N;
   const int ncell = N / c->hm.nb;
@@ -369,6 +448,14 @@ int obs_resize(esp_ctx *c) {
   if (b.flags & ESP_OBS_SFACTOR) {
     if ((rc = dalloc(c, b.d_sf, (size_t)R * 3 * b.nq))) return rc;
     if ((rc = dalloc(c, b.d_sfn, 1))) return rc;
+    // site chunks: enough waves to fill the chip, at least 64 sites each
+    const int ng = (R + kSfRep - 1) / kSfRep, nqb = (b.nq + kSfQ - 1) / kSfQ;
+    int nch = (kSfWaves + ng * nqb - 1) / (ng * nqb);
+    nch = std::max(1, std::min({nch, kSfMaxChunks, (N + 63) / 64}));
+    b.sf_chunk = (N + nch - 1) / nch;
+    b.sf_nchunk = (N + b.sf_chunk - 1) / b.sf_chunk;
+    if ((rc = dalloc(c, b.d_sfspin, (size_t)ng * N * kSfRep * 3))) return rc;
+    if ((rc = dalloc(c, b.d_sfpart, (size_t)b.sf_nchunk * R * 6 * b.nq))) return rc;
   }
   if (c->trace_cap > 0 && (b.flags & (ESP_OBS_ORDER | ESP_OBS_STAGGERED)))
     if ((rc = dalloc(c, b.d_trace, (size_t)c->trace_cap * R * 2))) return rc;
@@ -399,11 +486,22 @@ int obs_enqueue(esp_ctx *c) {
     const size_t shm = (b.flags & ESP_OBS_PM) && o.pm_lds ? sizeof(unsigned) * o.ndim * o.nfreq : 0;
     hipLaunchKernelGGL(k_obs_cells, dim3(b.cblocks, c->R), dim3(kObsBlock), shm, c->stream, o);
   }
-  if (b.flags & ESP_OBS_SPINCORR)
-    hipLaunchKernelGGL(k_obs_spincorr, dim3(b.sblocks, c->R), dim3(kObsBlock), 0, c->stream, o);
-  if (b.flags & ESP_OBS_SFACTOR)
-    hipLaunchKernelGGL(k_obs_sfactor, dim3((b.nq + kSfBlock - 1) / kSfBlock, (c->R + kSfRep - 1) / kSfRep),
-                       dim3(kSfBlock), 0, c->stream, o);
+  if (b.flags & ESP_OBS_SPINCORR) {
+    const int rep = b.nsp == 1 ? 8 : b.nsp == 2 ? 4 : 2;
+    const dim3 grid(b.sblocks, (c->R + rep - 1) / rep);
+    switch (b.nsp) {
+      case 1: hipLaunchKernelGGL((k_obs_spincorr<1, 8>), grid, dim3(kObsBlock), 0, c->stream, o); break;
+      case 2: hipLaunchKernelGGL((k_obs_spincorr<2, 4>), grid, dim3(kObsBlock), 0, c->stream, o); break;
+      case 3: hipLaunchKernelGGL((k_obs_spincorr<3, 2>), grid, dim3(kObsBlock), 0, c->stream, o); break;
+      default: hipLaunchKernelGGL((k_obs_spincorr<4, 2>), grid, dim3(kObsBlock), 0, c->stream, o); break;
+    }
+  }
+  if (b.flags & ESP_OBS_SFACTOR) {
+    const int ng = (c->R + kSfRep - 1) / kSfRep;
+    hipLaunchKernelGGL(k_sf_pack, dim3((o.N + kObsBlock - 1) / kObsBlock, ng), dim3(kObsBlock), 0, c->stream, o);
+    hipLaunchKernelGGL(k_sf_main, dim3((b.nq + kSfQ - 1) / kSfQ, ng, b.sf_nchunk), dim3(kSfQ), 0, c->stream, o);
+    hipLaunchKernelGGL(k_sf_fin, dim3((b.nq + kObsBlock - 1) / kObsBlock, c->R), dim3(kObsBlock), 0, c->stream, o);
+  }
   hipLaunchKernelGGL(k_obs_final, dim3(c->R), dim3(kObsBlock), 0, c->stream, o);
   HIPCHK(c, hipGetLastError());
   return ESP_OK;
@@ -475,7 +573,7 @@ int esp_set_spincorr(esp_ctx *c, int nshell, int nspecies, const int *species, i
   int rc = need_model(c);
   if (rc) return rc;
   auto &b = c->obs;
-  dfree(b.d_species); dfree(b.d_scnt); dfree(b.d_snbr); dfree(b.d_snorm);
+  dfree(b.d_skind); dfree(b.d_soff); dfree(b.d_sbond); dfree(b.d_sw);
   b.flags &= ~ESP_OBS_SPINCORR;
   b.nshell = b.nsp = b.zs = 0;
   if (nshell == 0) return changed(c);
@@ -507,11 +605,124 @@ int esp_set_spincorr(esp_ctx *c, int nshell, int nspecies, const int *species, i
       for (int l = 0; l < ct[(size_t)s * N + i]; ++l) ++ck[sp[nb[((size_t)s * zs + l) * N + i]]];
       for (int x = 0; x < nspecies; ++x) norm[((size_t)s * nspecies + x) * nspecies + sp[i]] = ck[x];
     }
-  if ((rc = upload(c, b.d_species, sp))) return rc;
-  if ((rc = upload(c, b.d_scnt, ct))) return rc;
-  if ((rc = upload(c, b.d_snbr, nb))) return rc;
-  if ((rc = upload(c, b.d_snorm, norm))) return rc;
-  b.nshell = nshell; b.nsp = nspecies; b.zs = zs;
+  // Bond entries of each shell: (owner, partner, owner's slot, weight owner->partner, weight
+  // partner->owner) with the weight 1 / ((n m_i) m_j) of the directed bond i -> j, n the table
+  // above read as (shell, kind i, kind j).  A shell whose table is symmetric as a multiset (j
+  // listed c times for i exactly when i is listed c times for j) pairs opposite directed bonds
+  // (the k-th listing of j by i with the k-th of i by j) into one entry owned by one of its
+  // sites, which adds both terms.  Owner: the site whose slot is lower (ties: the lower id),
+  // which on a translation-invariant table gives every site the same half of the slots, so
+  // a table row points the same way for all sites (coalesced gathers); if that leaves some
+  // site more entries than closed trails would (odd-degree sites joined to one extra vertex,
+  // a site owns half its bonds rounded up), the trails decide.  Any other shell keeps one
+  // entry per directed bond (reverse weight 0).
+  if (N > (int)kBondIdx) return fail(c, ESP_EINVAL, "esp_set_spincorr: too many sites");
+  const std::vector<int> &r2d = c->hm.ref2dev;
+  const std::vector<double> &mag = c->hm.mag;
+  auto wgt = [&](int s, int i, int j) {
+    return 1.0 / (norm[((size_t)s * nspecies + sp[i]) * nspecies + sp[j]] * mag[r2d[i]] * mag[r2d[j]]);
+  };
+  struct Ent { int o, p, l; double wf, wr; };
+  std::vector<std::vector<Ent>> ents(nshell);
+  for (int s = 0; s < nshell; ++s) {
+    using Key = std::pair<unsigned long long, int>;     // ((min id, max id), slot)
+    std::vector<Key> up, down;                          // slots of the lower / higher id
+    std::vector<std::pair<int, int>> self;
+    for (int i = 0; i < N; ++i)
+      for (int l = 0; l < ct[(size_t)s * N + i]; ++l) {
+        const int j = nb[((size_t)s * zs + l) * N + i];
+        if (i < j) up.push_back({(unsigned long long)i << 32 | (unsigned)j, l});
+        else if (i > j) down.push_back({(unsigned long long)j << 32 | (unsigned)i, l});
+        else self.push_back({i, l});
+      }
+    std::sort(up.begin(), up.end());
+    std::sort(down.begin(), down.end());
+    bool sym = up.size() == down.size();
+    for (size_t e = 0; sym && e < up.size(); ++e) sym = up[e].first == down[e].first;
+    std::vector<Ent> &es = ents[s];
+    if (sym) {
+      const int E = (int)up.size();
+      std::vector<int> eu(E), ev(E), deg(N + 1, 0), owner(E), nown(N, 0);
+      for (const auto &x : self) ++nown[x.first];
+      std::vector<int> nself(nown);
+      for (int e = 0; e < E; ++e) {
+        eu[e] = (int)(up[e].first >> 32); ev[e] = (int)(up[e].first & 0xffffffffu);
+        ++deg[eu[e]]; ++deg[ev[e]];
+        owner[e] = down[e].second < up[e].second ? ev[e] : eu[e];
+        ++nown[owner[e]];
+      }
+      const int rows_slot = N ? *std::max_element(nown.begin(), nown.end()) : 0;
+      int rows_trail = 0;
+      for (int v = 0; v < N; ++v) rows_trail = std::max(rows_trail, nself[v] + (deg[v] + 1) / 2);
+      if (rows_slot > rows_trail) {
+        for (int v = 0; v < N; ++v)
+          if (deg[v] & 1) { eu.push_back(v); ev.push_back(N); ++deg[v]; ++deg[N]; }
+        const int Et = (int)eu.size();
+        std::vector<int> start(N + 2, 0), adj(2 * (size_t)Et), tr(Et, -1);
+        for (int v = 0; v <= N; ++v) start[v + 1] = start[v] + deg[v];
+        std::vector<int> fill(start.begin(), start.end() - 1);
+        for (int e = 0; e < Et; ++e) { adj[fill[eu[e]]++] = e; adj[fill[ev[e]]++] = e; }
+        std::vector<int> ptr(start.begin(), start.end() - 1);
+        for (int v0 = 0; v0 <= N; ++v0) {  // every degree is even: a trail from v0 ends at v0
+          int cur = v0;
+          for (;;) {
+            int e = -1;
+            while (ptr[cur] < start[cur + 1]) {
+              const int x = adj[ptr[cur]++];
+              if (tr[x] < 0) { e = x; break; }
+            }
+            if (e < 0) break;
+            tr[e] = cur;
+            cur = eu[e] == cur ? ev[e] : eu[e];
+          }
+        }
+        for (int e = 0; e < E; ++e) owner[e] = tr[e];
+      }
+      for (int e = 0; e < E; ++e) {
+        const bool lo = owner[e] == eu[e];
+        const int o = owner[e], p = lo ? ev[e] : eu[e];
+        es.push_back({o, p, lo ? up[e].second : down[e].second, wgt(s, o, p), wgt(s, p, o)});
+      }
+    } else {
+      for (int i = 0; i < N; ++i)
+        for (int l = 0; l < ct[(size_t)s * N + i]; ++l) {
+          const int j = nb[((size_t)s * zs + l) * N + i];
+          if (i != j) es.push_back({i, j, l, wgt(s, i, j), 0.0});
+        }
+    }
+    for (const auto &x : self) es.push_back({x.first, x.first, x.second, wgt(s, x.first, x.first), 0.0});
+    std::sort(es.begin(), es.end(), [](const Ent &a, const Ent &b) { return a.o != b.o ? a.o < b.o : a.l < b.l; });
+  }
+  // Device-order tables: shell s has rows soff[s] .. soff[s+1]-1, as many as its largest
+  // number of entries per site; a row holds the partner's device index with its kind in the
+  // top bits and the weights (one kind: their sum; more kinds: forward, then reverse after
+  // all rows).  Slots past a site's entries point at the site itself with weights 0.
+  std::vector<int> kind(N), off(nshell + 1, 0);
+  for (int i = 0; i < N; ++i) kind[r2d[i]] = sp[i];
+  std::vector<std::vector<int>> per(nshell, std::vector<int>(N, 0));
+  for (int s = 0; s < nshell; ++s) {
+    for (const Ent &x : ents[s]) ++per[s][x.o];
+    off[s + 1] = off[s] + *std::max_element(per[s].begin(), per[s].end());
+  }
+  const size_t rows = (size_t)std::max(off[nshell], 1);
+  std::vector<unsigned> bond(rows * N);
+  std::vector<double> wt((nspecies > 1 ? 2 : 1) * rows * N, 0.0);
+  for (int s = 0; s < nshell; ++s) {
+    for (int i = 0; i < N; ++i)
+      for (int l = off[s]; l < off[s + 1]; ++l) bond[(size_t)l * N + r2d[i]] = (unsigned)r2d[i] | (unsigned)sp[i] << kBondKindShift;
+    std::vector<int> used(N, 0);
+    for (const Ent &x : ents[s]) {
+      const size_t t = (size_t)(off[s] + used[x.o]++) * N + r2d[x.o];
+      bond[t] = (unsigned)r2d[x.p] | (unsigned)sp[x.p] << kBondKindShift;
+      if (nspecies > 1) { wt[t] = x.wf; wt[rows * N + t] = x.wr; }
+      else wt[t] = x.wf + x.wr;
+    }
+  }
+  if ((rc = upload(c, b.d_skind, kind))) return rc;
+  if ((rc = upload(c, b.d_soff, off))) return rc;
+  if ((rc = upload(c, b.d_sbond, bond))) return rc;
+  if ((rc = upload(c, b.d_sw, wt))) return rc;
+  b.nshell = nshell; b.nsp = nspecies; b.zs = zs; b.srows = (int)rows;
   b.flags |= ESP_OBS_SPINCORR;
   return changed(c);
 }
@@ -525,7 +736,10 @@ int esp_set_sfactor(esp_ctx *c, int nq, const double *q, const double *pos, cons
   b.nq = 0;
   if (nq == 0) return changed(c);
   if (nq < 0 || !q || !pos || every < 1) return fail(c, ESP_EINVAL, "esp_set_sfactor: bad arguments");
-  std::vector<double> qv(q, q + 3 * (size_t)nq), pv(pos, pos + 3 * (size_t)c->hm.N);
+  const int N = c->hm.N;
+  std::vector<double> qv(q, q + 3 * (size_t)nq), pv(3 * (size_t)N);
+  for (int i = 0; i < N; ++i)
+    for (int k = 0; k < 3; ++k) pv[3 * (size_t)c->hm.ref2dev[i] + k] = pos[3 * (size_t)i + k];
   if ((rc = upload(c, b.d_q, qv))) return rc;
   if ((rc = upload(c, b.d_pos, pv))) return rc;
   b.nq = nq;

@@ -129,3 +129,108 @@ def test_observables_off_keeps_lds_path_and_errors_when_off():
     with pytest.raises(EspError):
         eng.obs(4, (2, 1, 201))
     eng.close()
+
+
+def _drop_bond(cnt, nbr, s, i, l):
+    """Remove slot l of site i (0-based) from shell s of the 1-based tables."""
+    n = cnt[s, i]
+    nbr[s, l:n - 1, i] = nbr[s, l + 1:n, i].copy()
+    nbr[s, n - 1, i] = 0
+    cnt[s, i] = n - 1
+
+
+@pytest.mark.parametrize("nspecies,tables", [(1, "full"), (1, "pairs"), (1, "ragged"), (1, "shuffled"),
+                                             (2, "pairs"), (3, "ragged"), (3, "shuffled"), (4, "full"),
+                                             (4, "ragged")])
+def test_spincorr_kinds_moments_tables(nspecies, tables):
+    """Spin correlations with 1-4 kinds (kind = basis atom mod nspecies), unequal moments and
+    three shells of different widths (12, 6, 24 on FCC 3^3, where a site lists some partners
+    more than once), on three kinds of tables: 'full' (symmetric: both directions of a bond
+    come from one entry; the owner of a bond is the site whose slot for it is lower, half the
+    slots of every site), 'shuffled' (symmetric, each site's list in random order: owners
+    from closed trails), 'pairs' (symmetric with odd counts: both directions of a fifth of
+    the bonds removed) and 'ragged' (not symmetric: single directions removed, one entry per
+    directed bond).  The last site of each kind, whose counts give the shell norms, keeps its
+    bonds.  Five replicas, so the last replica group is partly empty."""
+    from espins_amd.engine import Engine
+    L, R = 3, 5
+    m = Lt.cubic_model("fcc", L, J=(1.0, 0.2))
+    m.mag = np.tile([1.0, 1.5, 0.8, 2.0], L ** 3)
+    cnt, nbr = Lt.shell_tables("fcc", L, 3)
+    kinds = (np.arange(m.N) % m.nb % nspecies).astype(np.int32)
+    rng = np.random.default_rng(nspecies)
+    keep = np.zeros(m.N, bool)
+    keep[[np.flatnonzero(kinds == x)[-1] for x in range(nspecies)]] = True
+    if tables == "ragged":
+        cut = (rng.random(cnt.shape) < 0.2) & ~keep[None, :]
+        cnt = np.where(cut, rng.integers(0, cnt + 1), cnt).astype(np.int32)
+    elif tables == "pairs":
+        for s in range(3):
+            for i in range(m.N):
+                l = 0
+                while l < cnt[s, i]:
+                    j = nbr[s, l, i] - 1
+                    if i < j and not keep[i] and not keep[j] and rng.random() < 0.2:
+                        _drop_bond(cnt, nbr, s, i, l)
+                        back = int(np.flatnonzero(nbr[s, :cnt[s, j], j] == i + 1)[0])
+                        _drop_bond(cnt, nbr, s, j, back)
+                    else:
+                        l += 1
+        assert (cnt % 2 == 1).any()
+    elif tables == "shuffled":
+        for s in range(3):
+            for i in range(m.N):
+                nbr[s, :cnt[s, i], i] = rng.permutation(nbr[s, :cnt[s, i], i])
+    eng = Engine(0)
+    eng.set_model(m)
+    eng.set_replicas(Lt.ladder("lin", 0.5, 2.0, R), Lt.random_spins(m.N, R, 9, m.mag))
+    eng.set_spincorr(kinds, cnt, nbr)
+    ns = nspecies
+    sc = np.zeros((R, 3, 3, ns, ns))
+    for k in range(3):
+        say(f"nspecies {ns} tables {tables} step {k}")
+        slot_of = np.argsort(eng.pt_map()[0])
+        eng.run(1, steps_per_round=1, steps_measure=1, measure=True, exchange=True,
+                start_parity=1 + k % 2, seed=4, sweep0=k, step0=k)
+        s_all = eng.spins(by_slot=False)
+        for r in range(R):
+            t, ta = B.spincorr(s_all[r], kinds, cnt, nbr, m.mag, ns)
+            sc[slot_of[r], 0] += t
+            sc[slot_of[r], 1] += np.abs(t)
+            sc[slot_of[r], 2] += ta
+    got, n = eng.obs(8, (R, 3, 3, ns, ns))
+    eng.close()
+    assert n == 3
+    assert np.isfinite(sc).all()
+    assert np.allclose(got, sc, rtol=1e-12, atol=1e-12 * m.N)
+
+
+def test_sfactor_groups_chunks_and_partial_tiles():
+    """S(q) with 17 replicas (three groups of eight, the last one partly empty), 130 q-points
+    (three waves of 64, the last one partly empty, q = 0 among them) and 256 sites in four
+    site chunks, evaluated at every measurement, against the oracle's per-configuration sums."""
+    from espins_amd.engine import Engine
+    L, R = 4, 17
+    m = Lt.cubic_model("fcc", L, J=(1.0,))
+    eng = Engine(0)
+    eng.set_model(m)
+    eng.set_replicas(Lt.ladder("log", 0.3, 3.0, R), Lt.random_spins(m.N, R, 21, m.mag))
+    tp = 2 * math.pi
+    rng = np.random.default_rng(8)
+    q = np.concatenate([[(0.0, 0.0, 0.0), (tp, 0, 0)], rng.uniform(-3 * tp, 3 * tp, size=(128, 3))])
+    pos = Lt.site_positions("fcc", L)
+    polar = np.array([0.0, 0.6, -0.8])
+    eng.set_sfactor(q, pos, polar, every=1)
+    sf = np.zeros((R, 3, len(q)))
+    for k in range(2):
+        say(f"S(q) step {k}")
+        slot_of = np.argsort(eng.pt_map()[0])
+        eng.run(1, steps_per_round=1, steps_measure=1, measure=True, exchange=True,
+                start_parity=1 + k % 2, seed=2, sweep0=k, step0=k)
+        s_all = eng.spins(by_slot=False)
+        for r in range(R):
+            sf[slot_of[r]] += B.sfactor(s_all[r], pos, q, polar)
+    got, n = eng.obs(16, (R, 3, len(q)))
+    eng.close()
+    assert n == 2
+    assert np.allclose(got, sf, rtol=1e-10, atol=1e-12 * float(m.N) ** 2)
