@@ -103,6 +103,8 @@ struct esp_ctx {
     double *d_sfspin = nullptr;    // [replica groups][N][8][3] spins of an S(q) evaluation
     double *d_sfpart = nullptr;    // [site chunks][R][6][nq] partial F0
     int sf_chunk = 0, sf_nchunk = 0;
+    int sf_lat = 0, sfL[3] = {0, 0, 0};   // lattice positions: phases from per-q tables
+    double2 *d_sftab = nullptr;    // [L0 + L1 + L2 + nb][nq] exp(i q.x A_k), exp(i q.B_a)
     double *d_sf = nullptr;        // [R][3][nq]
     unsigned long long *d_sfn = nullptr;      // structure-factor measurements
     // per-measurement |OP| and |SM| by slot while the trace is on: [trace_cap][R][2]

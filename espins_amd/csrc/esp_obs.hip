@@ -11,8 +11,9 @@
 // k_obs_cells (one thread per cell: the cell sums, block partials of OP and SM, the P(m)
 // counts through a histogram in LDS), k_obs_spincorr (one thread per site and several
 // replicas, device order), k_sf_pack / k_sf_main / k_sf_fin (S(q) as a phase-matrix times
-// spin-matrix product: one phase per q-point, site and 8 replicas) and k_obs_final (per
-// replica: fixed-order sums of the block partials, accumulation into the replica's slot).
+// spin-matrix product: one phase per q-point, site and 8 replicas; k_sf_lat for lattice
+// positions: phases from per-q tables) and k_obs_final (per replica: fixed-order sums of the
+// block partials, accumulation into the replica's slot).
 // Histogram counts are integers, so they do not depend on the order of the atomic additions;
 // every floating-point sum has a fixed order.
 #include <hip/hip_runtime.h>
@@ -58,6 +59,10 @@ struct ObsArgs {
   int have_polar, nq, every;
   double *sfspin, *sfpart;
   int sf_chunk, sf_nchunk;
+  // lattice positions r = B_a + x0 A0 + x1 A1 + x2 A2 (reference order ((x2 L1 + x1) L0 + x0) nb
+  // + a): phase tables exp(i q.x0 A0), exp(i q.x1 A1), exp(i q.x2 A2), exp(i q.B_a), [k][q] each
+  int sf_lat, sfL[3];
+  const double2 *sfex, *sfey, *sfez, *sfeb;
   double *sf;
   unsigned long long *sfn;
   double *acc;
@@ -213,7 +218,8 @@ __global__ __launch_bounds__(kObsBlock) void k_obs_spincorr(ObsArgs o) {
 // phase matrix [q][site] with the spin matrix [site][replica, component]; a phase is computed
 // once per (q, site) and kSfRep replicas.  Three launches, each a no-op unless this
 // measurement is one that evaluates S(q) (uniform over the grid):
-//   k_sf_pack   the spins, site-major in groups of kSfRep replicas: [group][d][kSfRep][3];
+//   k_sf_pack   the spins, site-major in groups of kSfRep replicas: [group][site][kSfRep][3]
+//               (device order; reference order for lattice positions);
 //   k_sf_main   one wave per (64 q-points, replica group, site chunk): lane = q-point, the
 //               site's position and its kSfRep x 3 spins are wave-uniform loads (SGPRs),
 //               one sincos per site serves kSfRep x 6 FMAs (the wave is bound by its FP64
@@ -229,10 +235,11 @@ __device__ __forceinline__ bool sf_due(const ObsArgs &o) {
 
 __global__ __launch_bounds__(kObsBlock) void k_sf_pack(ObsArgs o) {
   if (!sf_due(o)) return;
-  const int d = blockIdx.x * kObsBlock + threadIdx.x;
-  if (d >= o.N) return;
+  const int j = blockIdx.x * kObsBlock + threadIdx.x;    // device site, or reference site (lattice)
+  if (j >= o.N) return;
+  const int d = o.sf_lat ? gldi(o.ref2dev, j) : j;
   const int g = blockIdx.y;
-  double *out = o.sfspin + ((size_t)g * o.N + d) * (kSfRep * 3);
+  double *out = o.sfspin + ((size_t)g * o.N + j) * (kSfRep * 3);
 #pragma unroll
   for (int u = 0; u < kSfRep; ++u) {
     const int r = g * kSfRep + u;
@@ -268,6 +275,64 @@ __global__ __launch_bounds__(kSfQ) void k_sf_main(ObsArgs o) {
       }
   }
   if (iq >= o.nq) return;
+#pragma unroll
+  for (int u = 0; u < kSfRep; ++u) {
+    const int r = g * kSfRep + u;
+    if (r >= o.R) break;
+    double *out = o.sfpart + ((size_t)ch * o.R + r) * 6 * o.nq + iq;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      gsti(out, (size_t)k * o.nq, fr[u][k]);
+      gsti(out, (size_t)(3 + k) * o.nq, fi[u][k]);
+    }
+  }
+}
+
+// k_sf_main for lattice positions: the phase of site (x0, x1, x2, a) is the product of four
+// table entries (exp(i q.x0 A0) exp(i q.x1 A1) exp(i q.x2 A2) exp(i q.B_a)): per site one
+// complex product instead of a sincos.  A chunk is a range of (x1, x2) rows, sites in
+// reference order inside a row.
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 ld2(const double2 *p, size_t i) {
+  const double *d = reinterpret_cast<const double *>(p);
+  return make_double2(gldi(d, 2 * i), gldi(d, 2 * i + 1));
+}
+
+__global__ __launch_bounds__(kSfQ) void k_sf_lat(ObsArgs o) {
+  if (!sf_due(o)) return;
+  const int iq0 = blockIdx.x * kSfQ + threadIdx.x;
+  const int iq = min(iq0, o.nq - 1);                 // lanes past nq compute q nq-1, store nothing
+  const int g = blockIdx.y, ch = blockIdx.z;
+  const int L0 = o.sfL[0], L1 = o.sfL[1], nb = o.nb;
+  const int rows = L1 * o.sfL[2];
+  const int w0 = ch * o.sf_chunk, w1 = min(rows, w0 + o.sf_chunk);
+  double fr[kSfRep][3], fi[kSfRep][3];
+#pragma unroll
+  for (int u = 0; u < kSfRep; ++u)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { fr[u][k] = 0.0; fi[u][k] = 0.0; }
+  const double *spg = o.sfspin + (size_t)g * o.N * (kSfRep * 3);
+  for (int w = w0; w < w1; ++w) {
+    const double2 byz = cmul(ld2(o.sfey, (size_t)(w % L1) * o.nq + iq), ld2(o.sfez, (size_t)(w / L1) * o.nq + iq));
+    for (int x0 = 0; x0 < L0; ++x0) {
+      const double2 bx = cmul(ld2(o.sfex, (size_t)x0 * o.nq + iq), byz);
+      const double *sv0 = spg + ((size_t)w * L0 + x0) * nb * (kSfRep * 3);
+      for (int a = 0; a < nb; ++a) {
+        const double2 ph = cmul(bx, ld2(o.sfeb, (size_t)a * o.nq + iq));
+        const double *sv = sv0 + (size_t)a * (kSfRep * 3);
+#pragma unroll
+        for (int u = 0; u < kSfRep; ++u)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            fr[u][k] += sv[3 * u + k] * ph.x;
+            fi[u][k] += sv[3 * u + k] * ph.y;
+          }
+      }
+    }
+  }
+  if (iq0 >= o.nq) return;
 #pragma unroll
   for (int u = 0; u < kSfRep; ++u) {
     const int r = g * kSfRep + u;
@@ -395,6 +460,12 @@ ObsArgs obs_args(esp_ctx *c) {
   for (int k = 0; k < 3; ++k) o.polar[k] = b.polar[k];
   o.have_polar = b.have_polar; o.nq = b.nq; o.every = b.sf_every < 1 ? 1 : b.sf_every;
   o.sfspin = b.d_sfspin; o.sfpart = b.d_sfpart; o.sf_chunk = b.sf_chunk; o.sf_nchunk = b.sf_nchunk;
+  o.sf_lat = b.sf_lat;
+  for (int k = 0; k < 3; ++k) o.sfL[k] = b.sfL[k];
+  o.sfex = b.d_sftab;
+  o.sfey = o.sfex ? o.sfex + (size_t)b.sfL[0] * b.nq : nullptr;
+  o.sfez = o.sfey ? o.sfey + (size_t)b.sfL[1] * b.nq : nullptr;
+  o.sfeb = o.sfez ? o.sfez + (size_t)b.sfL[2] * b.nq : nullptr;
   o.sf = b.d_sf; o.sfn = b.d_sfn;
   o.acc = b.d_acc; o.trace = b.d_trace; o.trace_cap = c->trace_cap;
   return o;
@@ -422,7 +493,7 @@ void obs_free(esp_ctx *c) {
   dfree(b.d_axes); dfree(b.d_coeff); dfree(b.d_acc); dfree(b.d_part); dfree(b.d_pm);
   dfree(b.d_skind); dfree(b.d_soff); dfree(b.d_sbond); dfree(b.d_sw); dfree(b.d_sc);
   dfree(b.d_scpart); dfree(b.d_q); dfree(b.d_pos); dfree(b.d_sf); dfree(b.d_sfn); dfree(b.d_trace);
-  dfree(b.d_sfspin); dfree(b.d_sfpart);
+  dfree(b.d_sfspin); dfree(b.d_sfpart); dfree(b.d_sftab);
   b = esp_ctx::Obs{};
 }
 
@@ -448,12 +519,14 @@ int obs_resize(esp_ctx *c) {
   if (b.flags & ESP_OBS_SFACTOR) {
     if ((rc = dalloc(c, b.d_sf, (size_t)R * 3 * b.nq))) return rc;
     if ((rc = dalloc(c, b.d_sfn, 1))) return rc;
-    // site chunks: enough waves to fill the chip, at least 64 sites each
+    // site chunks (lattice: chunks of (x1, x2) rows): enough waves to fill the chip, at least
+    // 64 sites each
     const int ng = (R + kSfRep - 1) / kSfRep, nqb = (b.nq + kSfQ - 1) / kSfQ;
+    const int units = b.sf_lat ? b.sfL[1] * b.sfL[2] : N;
     int nch = (kSfWaves + ng * nqb - 1) / (ng * nqb);
-    nch = std::max(1, std::min({nch, kSfMaxChunks, (N + 63) / 64}));
-    b.sf_chunk = (N + nch - 1) / nch;
-    b.sf_nchunk = (N + b.sf_chunk - 1) / b.sf_chunk;
+    nch = std::max(1, std::min({nch, kSfMaxChunks, units, (N + 63) / 64}));
+    b.sf_chunk = (units + nch - 1) / nch;
+    b.sf_nchunk = (units + b.sf_chunk - 1) / b.sf_chunk;
     if ((rc = dalloc(c, b.d_sfspin, (size_t)ng * N * kSfRep * 3))) return rc;
     if ((rc = dalloc(c, b.d_sfpart, (size_t)b.sf_nchunk * R * 6 * b.nq))) return rc;
   }
@@ -499,7 +572,10 @@ int obs_enqueue(esp_ctx *c) {
   if (b.flags & ESP_OBS_SFACTOR) {
     const int ng = (c->R + kSfRep - 1) / kSfRep;
     hipLaunchKernelGGL(k_sf_pack, dim3((o.N + kObsBlock - 1) / kObsBlock, ng), dim3(kObsBlock), 0, c->stream, o);
-    hipLaunchKernelGGL(k_sf_main, dim3((b.nq + kSfQ - 1) / kSfQ, ng, b.sf_nchunk), dim3(kSfQ), 0, c->stream, o);
+    if (b.sf_lat)
+      hipLaunchKernelGGL(k_sf_lat, dim3((b.nq + kSfQ - 1) / kSfQ, ng, b.sf_nchunk), dim3(kSfQ), 0, c->stream, o);
+    else
+      hipLaunchKernelGGL(k_sf_main, dim3((b.nq + kSfQ - 1) / kSfQ, ng, b.sf_nchunk), dim3(kSfQ), 0, c->stream, o);
     hipLaunchKernelGGL(k_sf_fin, dim3((b.nq + kObsBlock - 1) / kObsBlock, c->R), dim3(kObsBlock), 0, c->stream, o);
   }
   hipLaunchKernelGGL(k_obs_final, dim3(c->R), dim3(kObsBlock), 0, c->stream, o);
@@ -731,9 +807,10 @@ int esp_set_sfactor(esp_ctx *c, int nq, const double *q, const double *pos, cons
   int rc = need_model(c);
   if (rc) return rc;
   auto &b = c->obs;
-  dfree(b.d_q); dfree(b.d_pos);
+  dfree(b.d_q); dfree(b.d_pos); dfree(b.d_sftab);
   b.flags &= ~ESP_OBS_SFACTOR;
   b.nq = 0;
+  b.sf_lat = 0;
   if (nq == 0) return changed(c);
   if (nq < 0 || !q || !pos || every < 1) return fail(c, ESP_EINVAL, "esp_set_sfactor: bad arguments");
   const int N = c->hm.N;
@@ -742,6 +819,58 @@ int esp_set_sfactor(esp_ctx *c, int nq, const double *q, const double *pos, cons
     for (int k = 0; k < 3; ++k) pv[3 * (size_t)c->hm.ref2dev[i] + k] = pos[3 * (size_t)i + k];
   if ((rc = upload(c, b.d_q, qv))) return rc;
   if ((rc = upload(c, b.d_pos, pv))) return rc;
+  // Lattice positions: with the supercell L0 x L1 x L2 (as given to esp_set_supercell, else
+  // a cube) and r_i = B_a + x0 A0 + x1 A1 + x2 A2 for every site to within a few rounding
+  // errors (A_k, B_a read off the cell at the origin and its neighbours), the phases factor
+  // into per-q tables, computed here once.
+  const int nb = c->hm.nb, nc = N / nb;
+  int L[3] = {0, 0, 0};
+  if (c->Lhint[0] > 0 && (long)c->Lhint[0] * c->Lhint[1] * c->Lhint[2] == nc) {
+    for (int k = 0; k < 3; ++k) L[k] = c->Lhint[k];
+  } else {
+    const int e = (int)std::lround(std::cbrt((double)nc));
+    if (e > 0 && (long)e * e * e == nc) L[0] = L[1] = L[2] = e;
+  }
+  if (L[0] > 0) {
+    auto at = [&](int x0, int x1, int x2, int a) { return pos + 3 * ((((size_t)x2 * L[1] + x1) * L[0] + x0) * nb + a); };
+    double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, pmax = 0.0;
+    for (int k = 0; k < 3; ++k)
+      if (L[k] > 1) {
+        const int x[3] = {k == 0, k == 1, k == 2};
+        for (int m = 0; m < 3; ++m) A[k][m] = at(x[0], x[1], x[2], 0)[m] - pos[m];
+      }
+    for (size_t t = 0; t < 3 * (size_t)N; ++t) pmax = std::max(pmax, std::fabs(pos[t]));
+    const double tol = 8.0 * 2.220446049250313e-16 * (1.0 + pmax);
+    bool lat = true;
+    for (int x2 = 0; x2 < L[2] && lat; ++x2)
+      for (int x1 = 0; x1 < L[1] && lat; ++x1)
+        for (int x0 = 0; x0 < L[0] && lat; ++x0)
+          for (int a = 0; a < nb && lat; ++a)
+            for (int m = 0; m < 3; ++m) {
+              const double r = pos[3 * (size_t)a + m] + x0 * A[0][m] + x1 * A[1][m] + x2 * A[2][m];
+              if (std::fabs(at(x0, x1, x2, a)[m] - r) > tol) { lat = false; break; }
+            }
+    if (lat) {
+      std::vector<double2> tab((size_t)(L[0] + L[1] + L[2] + nb) * nq);
+      size_t row = 0;
+      for (int k = 0; k < 3; ++k)
+        for (int x = 0; x < L[k]; ++x, ++row)
+          for (int iq = 0; iq < nq; ++iq) {
+            const double *qq = q + 3 * (size_t)iq;
+            const double ph = qq[0] * (x * A[k][0]) + qq[1] * (x * A[k][1]) + qq[2] * (x * A[k][2]);
+            tab[row * nq + iq] = make_double2(std::cos(ph), std::sin(ph));
+          }
+      for (int a = 0; a < nb; ++a, ++row)
+        for (int iq = 0; iq < nq; ++iq) {
+          const double *qq = q + 3 * (size_t)iq, *ba = pos + 3 * (size_t)a;
+          const double ph = qq[0] * ba[0] + qq[1] * ba[1] + qq[2] * ba[2];
+          tab[row * nq + iq] = make_double2(std::cos(ph), std::sin(ph));
+        }
+      if ((rc = upload(c, b.d_sftab, tab))) return rc;
+      b.sf_lat = 1;
+      for (int k = 0; k < 3; ++k) b.sfL[k] = L[k];
+    }
+  }
   b.nq = nq;
   b.sf_every = every;
   b.have_polar = polar != nullptr;

@@ -234,3 +234,44 @@ def test_sfactor_groups_chunks_and_partial_tiles():
     eng.close()
     assert n == 2
     assert np.allclose(got, sf, rtol=1e-10, atol=1e-12 * float(m.N) ** 2)
+
+
+@pytest.mark.parametrize("positions", ["lattice", "jittered"])
+def test_sfactor_sheared_supercell(positions):
+    """S(q) on a BCC 6 x 4 x 3 supercell with a sheared cell (rows (1, 0, 0), (1/2, 1, 0),
+    (1/4, -1/2, 1): every position exact in binary): 'lattice' positions take the per-q phase
+    tables (three site chunks of (x1, x2) rows), 'jittered' ones (1e-3 random offsets) the
+    sincos path; both against the oracle's per-configuration sums."""
+    from espins_amd.engine import Engine
+    Ls, R = (6, 4, 3), 9
+    bf = np.array(Lt.BASIS["bcc"], float)
+    nb = len(bf)
+    N = Ls[0] * Ls[1] * Ls[2] * nb
+    m = Lt.Model("bcc", Ls, nb, bf, np.ones(N),
+                 Lt.build_table(nb, bf, np.ones(nb), Ls, Lt.shell_rows("bcc", [1.0]), 1))
+    x2, x1, x0 = np.meshgrid(range(Ls[2]), range(Ls[1]), range(Ls[0]), indexing="ij")
+    cells = np.stack([x0.ravel(), x1.ravel(), x2.ravel()], axis=1).astype(float)
+    shear = np.array([[1.0, 0.0, 0.0], [0.5, 1.0, 0.0], [0.25, -0.5, 1.0]])
+    pos = ((cells[:, None, :] + bf[None, :, :]).reshape(-1, 3)) @ shear
+    rng = np.random.default_rng(5)
+    if positions == "jittered":
+        pos = pos + rng.uniform(-1e-3, 1e-3, pos.shape)
+    tp = 2 * math.pi
+    q = rng.uniform(-2 * tp, 2 * tp, size=(70, 3))
+    eng = Engine(0)
+    eng.set_model(m)
+    eng.set_replicas(Lt.ladder("lin", 0.4, 2.5, R), Lt.random_spins(N, R, 13, m.mag))
+    eng.set_sfactor(q, pos, None, every=1)
+    sf = np.zeros((R, 3, len(q)))
+    for k in range(2):
+        say(f"S(q) {positions} step {k}")
+        slot_of = np.argsort(eng.pt_map()[0])
+        eng.run(1, steps_per_round=1, steps_measure=1, measure=True, exchange=True,
+                start_parity=1 + k % 2, seed=6, sweep0=k, step0=k)
+        s_all = eng.spins(by_slot=False)
+        for r in range(R):
+            sf[slot_of[r]] += B.sfactor(s_all[r], pos, q, None)
+    got, n = eng.obs(16, (R, 3, len(q)))
+    eng.close()
+    assert n == 2
+    assert np.allclose(got, sf, rtol=1e-10, atol=1e-12 * float(N) ** 2)
