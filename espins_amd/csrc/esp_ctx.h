@@ -81,6 +81,7 @@ struct esp_ctx {
     double *d_axes = nullptr, *d_coeff = nullptr;
     double *d_acc = nullptr;       // [R][6]: |OP|, OP^2, OP^4, |SM|, SM^2, SM^4
     double *d_part = nullptr;      // [R][cell blocks][4]
+    int *d_cdev = nullptr;         // [nb][ncell] device sites of the cells, cells by first atom's device index
     int cblocks = 0;
     // P(m): grid m_min + k (m_max - m_min) / (nfreq - 1); counts [R][3][nfreq]
     int nfreq = 0;

@@ -28,6 +28,7 @@ namespace {
 using namespace espk;
 
 constexpr int kObsBlock = 256;
+constexpr int kCellsPerThread = 4;    // cells per thread of k_obs_cells
 constexpr int kMaxSpecies = 4;        // kinds in the spin-correlation tables
 constexpr int kBondKindShift = 30;    // spin-correlation bond: partner | kind of partner << 30
 constexpr unsigned kBondIdx = (1u << kBondKindShift) - 1u;
@@ -46,6 +47,7 @@ struct ObsArgs {
   const double *axes, *coeff;
   double *part;
   int cblocks;
+  const int *cdev;                    // [nb][ncell] device index of atom a of the k-th cell
   unsigned long long *pm;
   int nfreq, ndim, pm_lds;
   double m_min, m_max;
@@ -84,23 +86,29 @@ __device__ __forceinline__ int pm_bin(const ObsArgs &o, double x) {
   return k > o.nfreq - 1 ? o.nfreq - 1 : k;
 }
 
-// One thread per cell (nb consecutive reference sites).
+// kCellsPerThread cells per thread (nb consecutive reference sites each), cells in the order
+// of cdev: a block covers kObsBlock x kCellsPerThread cells, so the P(m) histogram in LDS is
+// cleared and flushed (one global atomic per non-empty bin) once per that many cells.
 __global__ __launch_bounds__(kObsBlock) void k_obs_cells(ObsArgs o) {
   extern __shared__ unsigned s_hist[];
   __shared__ double red[4 * (kObsBlock / 64)];
   const int r = blockIdx.y;
-  const int c = blockIdx.x * kObsBlock + threadIdx.x;
   const bool PM = o.flags & ESP_OBS_PM, OP = o.flags & ESP_OBS_ORDER, SM = o.flags & ESP_OBS_STAGGERED;
   const int nh = o.ndim * o.nfreq;
   if (PM && o.pm_lds) {
     for (int k = threadIdx.x; k < nh; k += kObsBlock) s_hist[k] = 0u;
     __syncthreads();
   }
-  V3 m = {0.0, 0.0, 0.0}, st = {0.0, 0.0, 0.0};
-  double op = 0.0;
-  if (c < o.ncell) {
+  const int slot = gldi(o.rep_slot, r);
+  unsigned long long *g = PM ? o.pm + (size_t)slot * nh : nullptr;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};     // OP, SM x, y, z of this thread's cells
+  for (int t = 0; t < kCellsPerThread; ++t) {
+    const int c = (blockIdx.x * kCellsPerThread + t) * kObsBlock + threadIdx.x;
+    if (c >= o.ncell) break;
+    V3 m = {0.0, 0.0, 0.0}, st = {0.0, 0.0, 0.0};
+    double op = 0.0;
     for (int a = 0; a < o.nb; ++a) {
-      const int d = gldi(o.ref2dev, (size_t)c * o.nb + a);
+      const int d = gldi(o.cdev, (size_t)a * o.ncell + c);
       const V3 s = spin_at(o, r, d);
       m.x += s.x; m.y += s.y; m.z += s.z;
       if (OP) op += s.x * gldi(o.axes, 3 * a) + s.y * gldi(o.axes, 3 * a + 1) + s.z * gldi(o.axes, 3 * a + 2);
@@ -109,30 +117,27 @@ __global__ __launch_bounds__(kObsBlock) void k_obs_cells(ObsArgs o) {
         st.x += w * s.x; st.y += w * s.y; st.z += w * s.z;
       }
     }
-  }
-  const int slot = gldi(o.rep_slot, r);
-  if (PM) {
-    unsigned long long *g = o.pm + (size_t)slot * nh;
-    int bins[3], nbin = 0;
-    if (c < o.ncell) {
+    v[0] += op; v[1] += st.x; v[2] += st.y; v[3] += st.z;
+    if (PM) {
+      int bins[3], nbin = 0;
       bins[nbin++] = pm_bin(o, sqrt(m.x * m.x + m.y * m.y + m.z * m.z));
       if (OP) bins[nbin++] = o.nfreq + pm_bin(o, op);
       const int off = (OP ? 2 : 1) * o.nfreq;
       if (SM) bins[nbin++] = off + pm_bin(o, sqrt(st.x * st.x + st.y * st.y + st.z * st.z));
-    }
-    if (o.pm_lds) {
-      for (int k = 0; k < nbin; ++k) atomicAdd(&s_hist[bins[k]], 1u);
-      __syncthreads();
-      for (int k = threadIdx.x; k < nh; k += kObsBlock) {
-        const unsigned v = s_hist[k];
-        if (v) gadd(g + k, (unsigned long long)v);
+      for (int k = 0; k < nbin; ++k) {
+        if (o.pm_lds) atomicAdd(&s_hist[bins[k]], 1u);
+        else gadd(g + bins[k], 1ull);
       }
-    } else {
-      for (int k = 0; k < nbin; ++k) gadd(g + bins[k], 1ull);
+    }
+  }
+  if (PM && o.pm_lds) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < nh; k += kObsBlock) {
+      const unsigned h = s_hist[k];
+      if (h) gadd(g + k, (unsigned long long)h);
     }
   }
   if (OP || SM) {
-    double v[4] = {op, st.x, st.y, st.z};
     block_sum<4>(v, red);
     if (threadIdx.x == 0)
       for (int k = 0; k < 4; ++k) gsti(o.part, ((size_t)r * o.cblocks + blockIdx.x) * 4 + k, v[k]);
@@ -450,7 +455,7 @@ ObsArgs obs_args(esp_ctx *c) {
   o.sp = c->d_sp; o.Ns = c->Ns; o.N = m.N; o.nb = m.nb; o.R = c->R; o.ncell = m.N / m.nb;
   o.flags = b.flags;
   o.ref2dev = c->d_ref2dev; o.rep_slot = c->d_rep_slot; o.mag = c->d_mag; o.nmeas = c->d_nmeas;
-  o.axes = b.d_axes; o.coeff = b.d_coeff; o.part = b.d_part; o.cblocks = b.cblocks;
+  o.axes = b.d_axes; o.coeff = b.d_coeff; o.part = b.d_part; o.cblocks = b.cblocks; o.cdev = b.d_cdev;
   o.pm = b.d_pm; o.nfreq = b.nfreq; o.ndim = ndim_pm(c); o.m_min = b.m_min; o.m_max = b.m_max;
   o.pm_lds = o.ndim * o.nfreq <= kPmLdsBins;
   o.skind = b.d_skind; o.soff = b.d_soff; o.sbond = b.d_sbond; o.sw = b.d_sw; o.srows = b.srows;
@@ -493,20 +498,32 @@ void obs_free(esp_ctx *c) {
   dfree(b.d_axes); dfree(b.d_coeff); dfree(b.d_acc); dfree(b.d_part); dfree(b.d_pm);
   dfree(b.d_skind); dfree(b.d_soff); dfree(b.d_sbond); dfree(b.d_sw); dfree(b.d_sc);
   dfree(b.d_scpart); dfree(b.d_q); dfree(b.d_pos); dfree(b.d_sf); dfree(b.d_sfn); dfree(b.d_trace);
-  dfree(b.d_sfspin); dfree(b.d_sfpart); dfree(b.d_sftab);
+  dfree(b.d_sfspin); dfree(b.d_sfpart); dfree(b.d_sftab); dfree(b.d_cdev);
   b = esp_ctx::Obs{};
 }
 
 int obs_resize(esp_ctx *c) {
   auto &b = c->obs;
   dfree(b.d_acc); dfree(b.d_part); dfree(b.d_pm); dfree(b.d_sc); dfree(b.d_scpart);
-  dfree(b.d_sf); dfree(b.d_sfn); dfree(b.d_trace); dfree(b.d_sfspin); dfree(b.d_sfpart);
+  dfree(b.d_sf); dfree(b.d_sfn); dfree(b.d_trace); dfree(b.d_sfspin); dfree(b.d_sfpart); dfree(b.d_cdev);
   if (!b.flags || c->R == 0) return ESP_OK;
-  const int R = c->R, N = c->hm.N;
-  const int ncell = N / c->hm.nb;
-  b.cblocks = (ncell + kObsBlock - 1) / kObsBlock;
-  b.sblocks = (N + kObsBlock - 1) / kObsBlock;
+  const int R = c->R, N = c->hm.N, nb = c->hm.nb;
+  const int ncell = N / nb;
   int rc;
+  if (b.flags & (ESP_OBS_ORDER | ESP_OBS_STAGGERED | ESP_OBS_PM)) {
+    // Cells in the device order of their first atom: the device order groups the sites of a
+    // colour by basis atom and cell residue, cells in order inside a group, so atom a of
+    // consecutive cells in this order sits at consecutive device indices (coalesced gathers).
+    const std::vector<int> &r2d = c->hm.ref2dev;
+    std::vector<int> ord(ncell), cd((size_t)nb * ncell);
+    for (int k = 0; k < ncell; ++k) ord[k] = k;
+    std::sort(ord.begin(), ord.end(), [&](int x, int y) { return r2d[(size_t)x * nb] < r2d[(size_t)y * nb]; });
+    for (int a = 0; a < nb; ++a)
+      for (int k = 0; k < ncell; ++k) cd[(size_t)a * ncell + k] = r2d[(size_t)ord[k] * nb + a];
+    if ((rc = upload(c, b.d_cdev, cd))) return rc;
+  }
+  b.cblocks = (ncell + kObsBlock * kCellsPerThread - 1) / (kObsBlock * kCellsPerThread);
+  b.sblocks = (N + kObsBlock - 1) / kObsBlock;
   if ((rc = dalloc(c, b.d_acc, (size_t)R * 6))) return rc;
   if ((rc = dalloc(c, b.d_part, (size_t)R * b.cblocks * 4))) return rc;
   if (b.flags & ESP_OBS_PM)
