@@ -101,7 +101,7 @@ struct esp_ctx {
     double polar[3] = {0, 0, 0};
     int have_polar = 0;
     double *d_q = nullptr, *d_pos = nullptr;   // [nq][3], [N][3] by device site
-    double *d_sfspin = nullptr;    // [replica groups][N][8][3] spins of an S(q) evaluation
+    double *d_sfspin = nullptr;    // [replica groups][N][8 or 4][3] spins of an S(q) evaluation
     double *d_sfpart = nullptr;    // [site chunks][R][6][nq] partial F0
     int sf_chunk = 0, sf_nchunk = 0;
     int sf_lat = 0, sfL[3] = {0, 0, 0};   // lattice positions: phases from per-q tables

@@ -33,7 +33,8 @@ constexpr int kMaxSpecies = 4;        // kinds in the spin-correlation tables
 constexpr int kBondKindShift = 30;    // spin-correlation bond: partner | kind of partner << 30
 constexpr unsigned kBondIdx = (1u << kBondKindShift) - 1u;
 constexpr int kSfQ = 64;              // q-points per structure-factor wave (one per lane)
-constexpr int kSfRep = 8;             // replicas per structure-factor wave
+constexpr int kSfRep = 8;             // replicas per structure-factor wave (sincos path)
+constexpr int kSfRepLat = 4;          // replicas per structure-factor wave (lattice path)
 constexpr int kSfWaves = 4096;        // structure-factor waves aimed at (site chunks fill up to it)
 constexpr int kSfMaxChunks = 64;
 constexpr int kPmLdsBins = 12288;     // P(m) bins counted in LDS (48 KiB); more use global atomics
@@ -63,7 +64,7 @@ struct ObsArgs {
   int sf_chunk, sf_nchunk;
   // lattice positions r = B_a + x0 A0 + x1 A1 + x2 A2 (reference order ((x2 L1 + x1) L0 + x0) nb
   // + a): phase tables exp(i q.x0 A0), exp(i q.x1 A1), exp(i q.x2 A2), exp(i q.B_a), [k][q] each
-  int sf_lat, sfL[3];
+  int sf_lat, sfL[3], sf_rep;         // sf_rep: replicas per group of the packed spins
   const double2 *sfex, *sfey, *sfez, *sfeb;
   double *sf;
   unsigned long long *sfn;
@@ -224,7 +225,7 @@ __global__ __launch_bounds__(kObsBlock) void k_obs_spincorr(ObsArgs o) {
 // once per (q, site) and kSfRep replicas.  Three launches, each a no-op unless this
 // measurement is one that evaluates S(q) (uniform over the grid):
 //   k_sf_pack   the spins, site-major in groups of kSfRep replicas: [group][site][kSfRep][3]
-//               (device order; reference order for lattice positions);
+//               (device order; for lattice positions reference order and groups of kSfRepLat);
 //   k_sf_main   one wave per (64 q-points, replica group, site chunk): lane = q-point, the
 //               site's position and its kSfRep x 3 spins are wave-uniform loads (SGPRs),
 //               one sincos per site serves kSfRep x 6 FMAs (the wave is bound by its FP64
@@ -244,10 +245,11 @@ __global__ __launch_bounds__(kObsBlock) void k_sf_pack(ObsArgs o) {
   if (j >= o.N) return;
   const int d = o.sf_lat ? gldi(o.ref2dev, j) : j;
   const int g = blockIdx.y;
-  double *out = o.sfspin + ((size_t)g * o.N + j) * (kSfRep * 3);
+  double *out = o.sfspin + ((size_t)g * o.N + j) * (o.sf_rep * 3);
 #pragma unroll
   for (int u = 0; u < kSfRep; ++u) {
-    const int r = g * kSfRep + u;
+    if (u >= o.sf_rep) break;
+    const int r = g * o.sf_rep + u;
     const V3 s = r < o.R ? spin_at(o, r, d) : V3{0.0, 0.0, 0.0};
     gsti(out, 3 * u, s.x); gsti(out, 3 * u + 1, s.y); gsti(out, 3 * u + 2, s.z);
   }
@@ -306,6 +308,7 @@ __device__ __forceinline__ double2 ld2(const double2 *p, size_t i) {
 }
 
 __global__ __launch_bounds__(kSfQ) void k_sf_lat(ObsArgs o) {
+  constexpr int RV = kSfRepLat * 3;                 // spin values of one site and group
   if (!sf_due(o)) return;
   const int iq0 = blockIdx.x * kSfQ + threadIdx.x;
   const int iq = min(iq0, o.nq - 1);                 // lanes past nq compute q nq-1, store nothing
@@ -313,34 +316,46 @@ __global__ __launch_bounds__(kSfQ) void k_sf_lat(ObsArgs o) {
   const int L0 = o.sfL[0], L1 = o.sfL[1], nb = o.nb;
   const int rows = L1 * o.sfL[2];
   const int w0 = ch * o.sf_chunk, w1 = min(rows, w0 + o.sf_chunk);
-  double fr[kSfRep][3], fi[kSfRep][3];
+  double fr[kSfRepLat][3], fi[kSfRepLat][3];
 #pragma unroll
-  for (int u = 0; u < kSfRep; ++u)
+  for (int u = 0; u < kSfRepLat; ++u)
 #pragma unroll
     for (int k = 0; k < 3; ++k) { fr[u][k] = 0.0; fi[u][k] = 0.0; }
-  const double *spg = o.sfspin + (size_t)g * o.N * (kSfRep * 3);
+  auto add = [&](const double (&v)[RV], double2 ph) {
+#pragma unroll
+    for (int u = 0; u < kSfRepLat; ++u)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        fr[u][k] += v[3 * u + k] * ph.x;
+        fi[u][k] += v[3 * u + k] * ph.y;
+      }
+  };
+  const double *spg = o.sfspin + (size_t)g * o.N * RV;
   for (int w = w0; w < w1; ++w) {
     const double2 byz = cmul(ld2(o.sfey, (size_t)(w % L1) * o.nq + iq), ld2(o.sfez, (size_t)(w / L1) * o.nq + iq));
     for (int x0 = 0; x0 < L0; ++x0) {
       const double2 bx = cmul(ld2(o.sfex, (size_t)x0 * o.nq + iq), byz);
-      const double *sv0 = spg + ((size_t)w * L0 + x0) * nb * (kSfRep * 3);
-      for (int a = 0; a < nb; ++a) {
-        const double2 ph = cmul(bx, ld2(o.sfeb, (size_t)a * o.nq + iq));
-        const double *sv = sv0 + (size_t)a * (kSfRep * 3);
+      const double *sv0 = spg + ((size_t)w * L0 + x0) * nb * RV;
+      int a = 0;
+      for (; a + 1 < nb; a += 2) {     // two sites per wait on the (out-of-order) scalar loads
+        double v0[RV], v1[RV];
 #pragma unroll
-        for (int u = 0; u < kSfRep; ++u)
+        for (int k = 0; k < RV; ++k) { v0[k] = sv0[(size_t)a * RV + k]; v1[k] = sv0[(size_t)(a + 1) * RV + k]; }
+        add(v0, cmul(bx, ld2(o.sfeb, (size_t)a * o.nq + iq)));
+        add(v1, cmul(bx, ld2(o.sfeb, (size_t)(a + 1) * o.nq + iq)));
+      }
+      if (a < nb) {
+        double v0[RV];
 #pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            fr[u][k] += sv[3 * u + k] * ph.x;
-            fi[u][k] += sv[3 * u + k] * ph.y;
-          }
+        for (int k = 0; k < RV; ++k) v0[k] = sv0[(size_t)a * RV + k];
+        add(v0, cmul(bx, ld2(o.sfeb, (size_t)a * o.nq + iq)));
       }
     }
   }
   if (iq0 >= o.nq) return;
 #pragma unroll
-  for (int u = 0; u < kSfRep; ++u) {
-    const int r = g * kSfRep + u;
+  for (int u = 0; u < kSfRepLat; ++u) {
+    const int r = g * kSfRepLat + u;
     if (r >= o.R) break;
     double *out = o.sfpart + ((size_t)ch * o.R + r) * 6 * o.nq + iq;
 #pragma unroll
@@ -466,6 +481,7 @@ ObsArgs obs_args(esp_ctx *c) {
   o.have_polar = b.have_polar; o.nq = b.nq; o.every = b.sf_every < 1 ? 1 : b.sf_every;
   o.sfspin = b.d_sfspin; o.sfpart = b.d_sfpart; o.sf_chunk = b.sf_chunk; o.sf_nchunk = b.sf_nchunk;
   o.sf_lat = b.sf_lat;
+  o.sf_rep = b.sf_lat ? kSfRepLat : kSfRep;
   for (int k = 0; k < 3; ++k) o.sfL[k] = b.sfL[k];
   o.sfex = b.d_sftab;
   o.sfey = o.sfex ? o.sfex + (size_t)b.sfL[0] * b.nq : nullptr;
@@ -538,13 +554,14 @@ int obs_resize(esp_ctx *c) {
     if ((rc = dalloc(c, b.d_sfn, 1))) return rc;
     // site chunks (lattice: chunks of (x1, x2) rows): enough waves to fill the chip, at least
     // 64 sites each
-    const int ng = (R + kSfRep - 1) / kSfRep, nqb = (b.nq + kSfQ - 1) / kSfQ;
+    const int srep = b.sf_lat ? kSfRepLat : kSfRep;
+    const int ng = (R + srep - 1) / srep, nqb = (b.nq + kSfQ - 1) / kSfQ;
     const int units = b.sf_lat ? b.sfL[1] * b.sfL[2] : N;
     int nch = (kSfWaves + ng * nqb - 1) / (ng * nqb);
     nch = std::max(1, std::min({nch, kSfMaxChunks, units, (N + 63) / 64}));
     b.sf_chunk = (units + nch - 1) / nch;
     b.sf_nchunk = (units + b.sf_chunk - 1) / b.sf_chunk;
-    if ((rc = dalloc(c, b.d_sfspin, (size_t)ng * N * kSfRep * 3))) return rc;
+    if ((rc = dalloc(c, b.d_sfspin, (size_t)ng * N * srep * 3))) return rc;
     if ((rc = dalloc(c, b.d_sfpart, (size_t)b.sf_nchunk * R * 6 * b.nq))) return rc;
   }
   if (c->trace_cap > 0 && (b.flags & (ESP_OBS_ORDER | ESP_OBS_STAGGERED)))
@@ -587,7 +604,8 @@ int obs_enqueue(esp_ctx *c) {
     }
   }
   if (b.flags & ESP_OBS_SFACTOR) {
-    const int ng = (c->R + kSfRep - 1) / kSfRep;
+    const int srep = b.sf_lat ? kSfRepLat : kSfRep;
+    const int ng = (c->R + srep - 1) / srep;
     hipLaunchKernelGGL(k_sf_pack, dim3((o.N + kObsBlock - 1) / kObsBlock, ng), dim3(kObsBlock), 0, c->stream, o);
     if (b.sf_lat)
       hipLaunchKernelGGL(k_sf_lat, dim3((b.nq + kSfQ - 1) / kSfQ, ng, b.sf_nchunk), dim3(kSfQ), 0, c->stream, o);
