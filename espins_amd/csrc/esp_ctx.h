@@ -88,7 +88,7 @@ struct esp_ctx {
     double m_min = 0.0, m_max = 0.0;
     unsigned long long *d_pm = nullptr;
     // spin correlations per shell: bond tables by device site
-    int nshell = 0, nsp = 0, zs = 0, srows = 0;
+    int nshell = 0, nsp = 0, srows = 0;
     int *d_skind = nullptr;        // [N] kind of each site
     int *d_soff = nullptr;         // [nshell+1] first table row of each shell
     unsigned *d_sbond = nullptr;   // [srows][N] partner | kind of partner << 30

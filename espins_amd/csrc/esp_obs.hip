@@ -55,7 +55,7 @@ struct ObsArgs {
   const int *skind, *soff;
   const unsigned *sbond;
   const double *sw;
-  int nshell, nsp, zs, sblocks, srows;
+  int nshell, nsp, sblocks, srows;
   double *scpart, *sc;
   const double *q, *pos;               // pos by device site
   double polar[3];
@@ -474,7 +474,7 @@ ObsArgs obs_args(esp_ctx *c) {
   o.pm = b.d_pm; o.nfreq = b.nfreq; o.ndim = ndim_pm(c); o.m_min = b.m_min; o.m_max = b.m_max;
   o.pm_lds = o.ndim * o.nfreq <= kPmLdsBins;
   o.skind = b.d_skind; o.soff = b.d_soff; o.sbond = b.d_sbond; o.sw = b.d_sw; o.srows = b.srows;
-  o.nshell = b.nshell; o.nsp = b.nsp; o.zs = b.zs; o.sblocks = b.sblocks;
+  o.nshell = b.nshell; o.nsp = b.nsp; o.sblocks = b.sblocks;
   o.scpart = b.d_scpart; o.sc = b.d_sc;
   o.q = b.d_q; o.pos = b.d_pos;
   for (int k = 0; k < 3; ++k) o.polar[k] = b.polar[k];
@@ -686,7 +686,7 @@ int esp_set_spincorr(esp_ctx *c, int nshell, int nspecies, const int *species, i
   auto &b = c->obs;
   dfree(b.d_skind); dfree(b.d_soff); dfree(b.d_sbond); dfree(b.d_sw);
   b.flags &= ~ESP_OBS_SPINCORR;
-  b.nshell = b.nsp = b.zs = 0;
+  b.nshell = b.nsp = 0;
   if (nshell == 0) return changed(c);
   const int N = c->hm.N;
   if (nshell < 0 || nspecies < 1 || nspecies > kMaxSpecies || zs < 1 || !species || !cnt || !nbr)
@@ -833,7 +833,7 @@ int esp_set_spincorr(esp_ctx *c, int nshell, int nspecies, const int *species, i
   if ((rc = upload(c, b.d_soff, off))) return rc;
   if ((rc = upload(c, b.d_sbond, bond))) return rc;
   if ((rc = upload(c, b.d_sw, wt))) return rc;
-  b.nshell = nshell; b.nsp = nspecies; b.zs = zs; b.srows = (int)rows;
+  b.nshell = nshell; b.nsp = nspecies; b.srows = (int)rows;
   b.flags |= ESP_OBS_SPINCORR;
   return changed(c);
 }
