@@ -594,11 +594,11 @@ int obs_enqueue(esp_ctx *c) {
     hipLaunchKernelGGL(k_obs_cells, dim3(b.cblocks, c->R), dim3(kObsBlock), shm, c->stream, o);
   }
   if (b.flags & ESP_OBS_SPINCORR) {
-    const int rep = b.nsp == 1 ? 2 : b.nsp == 2 ? 4 : 2;   // one kind: 2 replicas beat 4 and 8
+    const int rep = 2;     // replicas per thread: 2 beat 4 and 8 (one kind and two kinds measured)
     const dim3 grid(b.sblocks, (c->R + rep - 1) / rep);
     switch (b.nsp) {
       case 1: hipLaunchKernelGGL((k_obs_spincorr<1, 2>), grid, dim3(kObsBlock), 0, c->stream, o); break;
-      case 2: hipLaunchKernelGGL((k_obs_spincorr<2, 4>), grid, dim3(kObsBlock), 0, c->stream, o); break;
+      case 2: hipLaunchKernelGGL((k_obs_spincorr<2, 2>), grid, dim3(kObsBlock), 0, c->stream, o); break;
       case 3: hipLaunchKernelGGL((k_obs_spincorr<3, 2>), grid, dim3(kObsBlock), 0, c->stream, o); break;
       default: hipLaunchKernelGGL((k_obs_spincorr<4, 2>), grid, dim3(kObsBlock), 0, c->stream, o); break;
     }
