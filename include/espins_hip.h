@@ -214,14 +214,18 @@ int esp_set_staggered(esp_ctx *ctx, int enable, const double *coeff);
  * (at least 2).  *nfreq receives the grid size. */
 int esp_set_pm(esp_ctx *ctx, int enable, int *nfreq);
 /* Spin correlations per shell (shell_neighbors_get, jij.F90:449-503): species[N] (0-based
- * kind of every site), cnt[nshell][N] and nbr[nshell][zs][N] (1-based site ids): the
- * neighbours of each site in each shell.  The normalisation is the reference's
- * shell_num_nbors_kind (its per-kind count of the last site of each kind). */
+ * kind of every site, at most 4 kinds), cnt[nshell][N] and nbr[nshell][zs][N] (1-based site
+ * ids): the neighbours of each site in each shell.  The normalisation is the reference's
+ * shell_num_nbors_kind (its per-kind count of the last site of each kind).  The call builds
+ * device-order bond tables on the host (a shell whose table is symmetric stores each bond
+ * once, with both directions' weights); the tables belong to the current model. */
 int esp_set_spincorr(esp_ctx *ctx, int nshell, int nspecies, const int *species, int zs,
                      const int *cnt, const int *nbr);
 /* Structure factor on nq q-points q[nq][3] (Cartesian, the phase is q . r) for site
  * positions pos[N][3] (Cartesian), measured when the measurement count (from 1) is a
- * multiple of every (sfactor_steps_measure); polar[3] (sfactor_polar) or NULL. */
+ * multiple of every (sfactor_steps_measure); polar[3] (sfactor_polar) or NULL.  Positions
+ * that form the supercell's lattice (esp_set_supercell, else a cube) get per-q phase tables
+ * at this call; others are evaluated with one sincos per (q, site) and 8 replicas. */
 int esp_set_sfactor(esp_ctx *ctx, int nq, const double *q, const double *pos, const double *polar,
                     int every);
 /* Accumulators by slot (sums over the measurements; divide by the counts):
